@@ -1,0 +1,164 @@
+// C-ABI layer of libcudilithium_amd.so: argument checking, pool layouts and
+// the host side of each entry point declared in include/cudilithium_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/cudilithium_amd.h"
+#include "internal.hpp"
+
+namespace dil {
+
+int device_cu_count() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cached[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached[dev] = cus;
+    }
+    return cached[dev];
+}
+
+int default_grid_blocks(size_t units, int per_block, int blocks_per_cu) {
+    size_t need = (units + per_block - 1) / per_block;
+    size_t cap = (size_t)device_cu_count() * blocks_per_cu;
+    if (need > cap) need = cap;
+    if (need == 0) need = 1;
+    return (int)need;
+}
+
+}  // namespace dil
+
+using namespace dil;
+
+static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+extern "C" {
+
+size_t dil_public_key_bytes(int mode) {
+    switch (mode) {
+        case 2: return Sizes<Params<2>>::PK_BYTES;
+        case 3: return Sizes<Params<3>>::PK_BYTES;
+        case 5: return Sizes<Params<5>>::PK_BYTES;
+        default: return 0;
+    }
+}
+
+size_t dil_secret_key_bytes(int mode) {
+    switch (mode) {
+        case 2: return Sizes<Params<2>>::SK_BYTES;
+        case 3: return Sizes<Params<3>>::SK_BYTES;
+        case 5: return Sizes<Params<5>>::SK_BYTES;
+        default: return 0;
+    }
+}
+
+size_t dil_signature_bytes(int mode) {
+    switch (mode) {
+        case 2: return Sizes<Params<2>>::SIG_BYTES;
+        case 3: return Sizes<Params<3>>::SIG_BYTES;
+        case 5: return Sizes<Params<5>>::SIG_BYTES;
+        default: return 0;
+    }
+}
+
+/* ---------------------------- NTT --------------------------------------- */
+
+int dil_ntt_forward(int32_t *d_poly, size_t npoly, size_t pitch_words, void *stream) {
+    if ((npoly && !d_poly) || pitch_words < (size_t)N || (pitch_words & 3)) return DIL_EINVAL;
+    return launch_ntt(0, d_poly, d_poly, nullptr, npoly, pitch_words, 0, S(stream));
+}
+
+int dil_ntt_inverse(int32_t *d_poly, size_t npoly, size_t pitch_words, void *stream) {
+    if ((npoly && !d_poly) || pitch_words < (size_t)N || (pitch_words & 3)) return DIL_EINVAL;
+    return launch_ntt(1, d_poly, d_poly, nullptr, npoly, pitch_words, 0, S(stream));
+}
+
+int dil_ntt_pointwise_pipeline(int32_t *d_c, const int32_t *d_a, const int32_t *d_b, size_t npoly,
+                               size_t pitch_words, void *stream) {
+    if ((npoly && (!d_c || !d_a || !d_b)) || pitch_words < (size_t)N || (pitch_words & 3)) return DIL_EINVAL;
+    return launch_ntt(2, d_c, d_a, d_b, npoly, pitch_words, 0, S(stream));
+}
+
+int dil_keccak_f1600_batch(uint64_t *d_states, size_t n, void *stream) {
+    if (n && !d_states) return DIL_EINVAL;
+    return launch_keccak(d_states, n, S(stream));
+}
+
+/* ---------------------------- runtime ----------------------------------- */
+
+int dil_device_count(int *count) { return (int)hipGetDeviceCount(count); }
+int dil_set_device(int device) { return (int)hipSetDevice(device); }
+
+int dil_device_name(char *buf, size_t len) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return (int)e;
+    if (buf && len) {
+        snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    }
+    return 0;
+}
+
+int dil_device_cu_count(void) { return device_cu_count(); }
+int dil_malloc_device(void **ptr, size_t bytes) { return (int)hipMalloc(ptr, bytes ? bytes : 1); }
+int dil_malloc_pitch(void **ptr, size_t *pitch, size_t width, size_t height) {
+    return (int)hipMallocPitch(ptr, pitch, width ? width : 1, height ? height : 1);
+}
+int dil_free_device(void *ptr) { return (int)hipFree(ptr); }
+int dil_malloc_host(void **ptr, size_t bytes) { return (int)hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault); }
+int dil_free_host(void *ptr) { return (int)hipHostFree(ptr); }
+int dil_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+    return bytes ? (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream)) : 0;
+}
+int dil_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+    return bytes ? (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(stream)) : 0;
+}
+int dil_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
+    return bytes ? (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(stream)) : 0;
+}
+int dil_memset_device(void *dst, int value, size_t bytes, void *stream) {
+    return bytes ? (int)hipMemsetAsync(dst, value, bytes, S(stream)) : 0;
+}
+int dil_stream_create(void **stream) {
+    hipStream_t s;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) *stream = s;
+    return (int)e;
+}
+int dil_stream_destroy(void *stream) { return (int)hipStreamDestroy(S(stream)); }
+int dil_stream_synchronize(void *stream) { return (int)hipStreamSynchronize(S(stream)); }
+int dil_device_synchronize(void) { return (int)hipDeviceSynchronize(); }
+int dil_event_create(void **event) {
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r == hipSuccess) *event = e;
+    return (int)r;
+}
+int dil_event_destroy(void *event) { return (int)hipEventDestroy(reinterpret_cast<hipEvent_t>(event)); }
+int dil_event_record(void *event, void *stream) {
+    return (int)hipEventRecord(reinterpret_cast<hipEvent_t>(event), S(stream));
+}
+int dil_event_synchronize(void *event) { return (int)hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)); }
+int dil_event_elapsed_ms(float *ms, void *start, void *stop) {
+    return (int)hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start), reinterpret_cast<hipEvent_t>(stop));
+}
+
+const char *dil_error_string(int code) {
+    switch (code) {
+        case DIL_OK: return "success";
+        case DIL_EINVAL: return "invalid argument";
+        case DIL_ENOMEM: return "caller-provided buffer too small";
+        case DIL_ERANGE: return "size beyond what the engine supports";
+        default: return code > 0 ? hipGetErrorString((hipError_t)code) : "unknown error";
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,27 @@
+// Host-side helpers shared by the kernel translation units and the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "keccak.hpp"
+#include "params.hpp"
+
+namespace dil {
+
+// Number of compute units of the current device (cached per device).
+int device_cu_count();
+
+// Grid size for a grid-stride kernel: enough blocks to cover `units` work
+// items at `per_block` items per block, capped at `waves_per_cu` resident
+// blocks per CU.
+int default_grid_blocks(size_t units, int per_block, int blocks_per_cu = 8);
+
+// Launchers (return a hipError_t value, 0 on success).
+int launch_ntt(int kind, int32_t *c, const int32_t *a, const int32_t *b, size_t npoly, size_t pitch_words,
+               int blocks, hipStream_t stream);
+int launch_keccak(uint64_t *st, size_t n, hipStream_t stream);
+
+}  // namespace dil
