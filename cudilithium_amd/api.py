@@ -1,0 +1,223 @@
+"""Python mirror of the reference's batched interface (include/api.cuh:27-63).
+
+Same names, argument meaning, pool ownership and return conventions as
+crypto_sign_keypair / crypto_sign_signature / crypto_sign_verify, over the
+C-ABI of libcudilithium_amd.so.  Host buffers are numpy arrays (pinned
+HostBuffer arrays work too); pools are caller-allocated pitched device
+buffers sized with the reference's formulas (reference_pool).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceBuffer, HostBuffer, PitchedBuffer, stream_sync
+
+c_void_p, c_size_t, c_int = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+u64p, u32p, i32p = ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int32)
+
+POOL_KEYPAIR, POOL_SIGN, POOL_SIGN_TEMP, POOL_VERIFY = 0, 1, 2, 3
+ENGINE_PERSISTENT, ENGINE_PERSTAGE = 0, 1
+
+
+class SignLutElement(ctypes.Structure):  # api.cuh:31-34
+    _fields_ = [("done_flag", ctypes.c_int), ("nonce", ctypes.c_uint16)]
+
+
+class TaskLutC(ctypes.Structure):  # api.cuh:41-49 (sign_lut as an array)
+    _fields_ = [("sign_lut", ctypes.POINTER(SignLutElement)), ("sign_lut_len", c_size_t),
+                ("h_exec_lut", c_void_p), ("h_done_lut", c_void_p), ("h_copy_lut", c_void_p),
+                ("d_exec_lut", c_void_p), ("d_done_lut", c_void_p), ("d_copy_lut", c_void_p)]
+
+
+_registered = False
+
+
+def _register():
+    global _registered
+    if _registered:
+        return
+    R = _lib.register
+    R("dil_keypair_pool_row_bytes", c_size_t, [c_int])
+    R("dil_sign_pool_row_bytes", c_size_t, [c_int, c_size_t])
+    R("dil_sign_temp_row_bytes", c_size_t, [c_int])
+    R("dil_verify_pool_row_bytes", c_size_t, [c_int, c_size_t])
+    R("dil_reference_pool_row_bytes", c_size_t, [c_int, c_int, c_size_t])
+    R("dil_crypto_sign_keypair", c_int, [c_int, c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, c_size_t])
+    sig_args = [c_int, c_void_p, c_size_t, ctypes.POINTER(c_size_t), c_void_p, c_size_t, c_size_t, c_void_p,
+                c_void_p, c_size_t, c_void_p, c_size_t, ctypes.POINTER(TaskLutC), c_size_t, c_size_t, c_void_p]
+    R("dil_crypto_sign_signature", c_int, sig_args)
+    R("dil_crypto_sign_signature_perstage", c_int, sig_args)
+    R("dil_crypto_sign_verify", c_int, [c_int, c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, c_size_t, c_size_t,
+                                        c_void_p, c_void_p, c_size_t, c_size_t, c_void_p])
+    R("dil_keypair_work_row_bytes", c_size_t, [c_int])
+    R("dil_keypair_device", c_int, [c_int, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t,
+                                    ctypes.c_uint64, c_size_t, c_void_p, c_size_t, c_void_p])
+    R("dil_verify_work_row_bytes", c_size_t, [c_int])
+    R("dil_verify_device", c_int, [c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p,
+                                   c_void_p, c_size_t, c_void_p, c_size_t, c_void_p])
+    _registered = True
+
+
+def L():
+    _register()
+    return _lib.load()
+
+
+def sizes(mode: int):
+    lib = L()
+    pk, sk, sig = lib.dil_public_key_bytes(mode), lib.dil_secret_key_bytes(mode), lib.dil_signature_bytes(mode)
+    if not pk:
+        raise ValueError(f"unknown Dilithium mode {mode}")
+    return pk, sk, sig
+
+
+def reference_row_bytes(mode: int, which: int, mlen: int = 0) -> int:
+    return L().dil_reference_pool_row_bytes(mode, which, mlen)
+
+
+def reference_pool(mode: int, which: int, rows: int, mlen: int = 0) -> PitchedBuffer:
+    """cudaMallocPitch of `rows` rows sized by the reference's formula."""
+    return PitchedBuffer(reference_row_bytes(mode, which, mlen), rows)
+
+
+def _hp(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+class TaskLut:
+    """task_lut as allocated by the reference's callers
+    (tests/test_vectors.cu:104-113): exec_threshold entries each."""
+
+    def __init__(self, batch: int, exec_threshold: int):
+        self.exec_threshold = exec_threshold
+        self.sign_lut = (SignLutElement * max(1, batch))()
+        self.h_exec = HostBuffer((exec_threshold,), np.uint32)
+        self.h_done = HostBuffer((exec_threshold,), np.uint8)
+        self.h_copy = HostBuffer((exec_threshold, 2), np.uint64)
+        self.d_exec = DeviceBuffer(4 * exec_threshold)
+        self.d_done = DeviceBuffer(exec_threshold)
+        self.d_copy = DeviceBuffer(16 * exec_threshold)
+        self.c = TaskLutC(ctypes.cast(self.sign_lut, ctypes.POINTER(SignLutElement)), batch, self.h_exec.ptr,
+                          self.h_done.ptr, self.h_copy.ptr, self.d_exec.ptr, self.d_done.ptr, self.d_copy.ptr)
+
+
+def crypto_sign_keypair(mode, pk: np.ndarray, sk: np.ndarray, d_pool: PitchedBuffer, pitch: int,
+                        batch_size: int = 1, stream=None, rand_index: int = 0) -> int:
+    r = L().dil_crypto_sign_keypair(mode, _hp(pk), _hp(sk), d_pool.ptr, pitch, batch_size, stream, rand_index)
+    _lib.check(r, "crypto_sign_keypair")
+    return r
+
+
+def crypto_sign_signature(mode, sig: np.ndarray, sig_pitch: int, m: np.ndarray, m_pitch: int, mlen: int,
+                          sk: np.ndarray, d_sign_pool: PitchedBuffer, sign_pitch: int,
+                          d_temp_pool: PitchedBuffer, temp_pitch: int, lut: TaskLut, exec_threshold: int,
+                          batch_size: int = 1, stream=None, engine: int = ENGINE_PERSISTENT) -> int:
+    """Returns siglen."""
+    fn = L().dil_crypto_sign_signature if engine == ENGINE_PERSISTENT else L().dil_crypto_sign_signature_perstage
+    siglen = c_size_t(0)
+    mp = _hp(m) if m is not None and m.size else None
+    r = fn(mode, _hp(sig), sig_pitch, ctypes.byref(siglen), mp, m_pitch, mlen, _hp(sk), d_sign_pool.ptr,
+           sign_pitch, d_temp_pool.ptr, temp_pitch, ctypes.byref(lut.c), exec_threshold, batch_size, stream)
+    _lib.check(r, "crypto_sign_signature")
+    return siglen.value
+
+
+def crypto_sign_verify(mode, ret: np.ndarray, sig: np.ndarray, sig_pitch: int, siglen: int, m: np.ndarray,
+                       m_pitch: int, mlen: int, pk: np.ndarray, d_pool: PitchedBuffer, pitch: int,
+                       batch_size: int = 1, stream=None) -> None:
+    assert ret.dtype == np.int32
+    mp = _hp(m) if m is not None and m.size else None
+    r = L().dil_crypto_sign_verify(mode, _hp(ret), _hp(sig), sig_pitch, siglen, mp, m_pitch, mlen, _hp(pk),
+                                   d_pool.ptr, pitch, batch_size, stream)
+    _lib.check(r, "crypto_sign_verify")
+
+
+class Dilithium:
+    """Convenience wrapper over the drop-in calls for one mode: allocates
+    pools sized by the reference's formulas and returns numpy arrays."""
+
+    def __init__(self, mode: int):
+        self.mode = mode
+        self.pk_bytes, self.sk_bytes, self.sig_bytes = sizes(mode)
+
+    def keypair(self, batch: int, rand_index: int = 0):
+        pool = reference_pool(self.mode, POOL_KEYPAIR, batch)
+        pk = np.zeros((batch, self.pk_bytes), np.uint8)
+        sk = np.zeros((batch, self.sk_bytes), np.uint8)
+        crypto_sign_keypair(self.mode, pk, sk, pool, pool.pitch, batch, None, rand_index)
+        stream_sync(None)
+        return pk, sk
+
+    def sign(self, m: np.ndarray, sk: np.ndarray, exec_threshold: int = 2048, engine: int = ENGINE_PERSISTENT):
+        m = np.ascontiguousarray(m, dtype=np.uint8)
+        batch, mlen = m.shape
+        pool = reference_pool(self.mode, POOL_SIGN, batch, mlen)
+        temp = reference_pool(self.mode, POOL_SIGN_TEMP, exec_threshold)
+        lut = TaskLut(batch, exec_threshold)
+        sig = np.zeros((batch, self.sig_bytes), np.uint8)
+        siglen = crypto_sign_signature(self.mode, sig, self.sig_bytes, m, mlen, mlen, np.ascontiguousarray(sk),
+                                       pool, pool.pitch, temp, temp.pitch, lut, exec_threshold, batch, None, engine)
+        stream_sync(None)
+        assert siglen == self.sig_bytes
+        return sig
+
+    def verify(self, sig: np.ndarray, m: np.ndarray, pk: np.ndarray, siglen: int | None = None):
+        m = np.ascontiguousarray(m, dtype=np.uint8)
+        batch, mlen = m.shape
+        pool = reference_pool(self.mode, POOL_VERIFY, batch, mlen)
+        ret = np.zeros(batch, np.int32)
+        crypto_sign_verify(self.mode, ret, np.ascontiguousarray(sig), self.sig_bytes,
+                           self.sig_bytes if siglen is None else siglen, m, mlen, mlen, np.ascontiguousarray(pk),
+                           pool, pool.pitch, batch, None)
+        stream_sync(None)
+        return ret
+
+
+# ---------------------------------------------------------------- device API
+
+def _pack_msgs(msgs):
+    """List of byte strings -> (flat uint8, offsets u64, lengths u32) with
+    8-byte aligned offsets."""
+    lens = np.array([len(m) for m in msgs], dtype=np.uint32)
+    offs = np.zeros(len(msgs), dtype=np.uint64)
+    pos = 0
+    for i, n in enumerate(lens):
+        offs[i] = pos
+        pos += (int(n) + 7) // 8 * 8
+    flat = np.zeros(max(8, pos), dtype=np.uint8)
+    for i, m in enumerate(msgs):
+        if len(m):
+            flat[int(offs[i]):int(offs[i]) + len(m)] = np.frombuffer(bytes(m), np.uint8)
+    return flat, offs, lens
+
+
+def keypair_device(mode: int, seeds: np.ndarray | None = None, n: int | None = None, rand_index: int = 0):
+    pkb, skb, _ = sizes(mode)
+    if seeds is not None:
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint8).reshape(-1, 32)
+        n = seeds.shape[0]
+    d_pk, d_sk = DeviceBuffer(n * pkb), DeviceBuffer(n * skb)
+    wrow = (L().dil_keypair_work_row_bytes(mode) + 255) // 256 * 256
+    d_work = DeviceBuffer(n * wrow)
+    d_seeds = DeviceBuffer.from_array(seeds) if seeds is not None else None
+    _lib.check(L().dil_keypair_device(mode, d_pk.ptr, pkb, d_sk.ptr, skb, d_seeds.ptr if d_seeds else None, 32,
+                                      rand_index, n, d_work.ptr, wrow, None), "keypair_device")
+    return d_pk.download((n, pkb), np.uint8), d_sk.download((n, skb), np.uint8)
+
+
+def verify_device(mode: int, sig: np.ndarray, msgs, pk: np.ndarray):
+    _, _, sigb = sizes(mode)
+    n = len(msgs)
+    flat, offs, lens = _pack_msgs(msgs)
+    d_sig, d_pk = DeviceBuffer.from_array(np.ascontiguousarray(sig)), DeviceBuffer.from_array(np.ascontiguousarray(pk))
+    d_msg, d_off, d_len = DeviceBuffer.from_array(flat), DeviceBuffer.from_array(offs), DeviceBuffer.from_array(lens)
+    wrow = L().dil_verify_work_row_bytes(mode)
+    d_work = DeviceBuffer(n * wrow)
+    d_ret = DeviceBuffer(4 * n)
+    _lib.check(L().dil_verify_device(mode, d_ret.ptr, d_sig.ptr, sig.shape[1], d_pk.ptr, pk.shape[1], d_msg.ptr,
+                                     d_off.ptr, d_len.ptr, n, d_work.ptr, wrow, None), "verify_device")
+    return d_ret.download((n,), np.int32)

@@ -151,6 +151,35 @@ int dil_event_elapsed_ms(float *ms, void *start, void *stop) {
     return (int)hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start), reinterpret_cast<hipEvent_t>(stop));
 }
 
+/* ---------------------------- synthetic inputs --------------------------- */
+
+// xoshiro256** (Blackman-Vigna) seeded through splitmix64; the workload
+// generator of the benchmark configurations.
+int dil_xoshiro_bytes(uint64_t seed, uint8_t *out, size_t n) {
+    if (n && !out) return DIL_EINVAL;
+    uint64_t x = seed, s[4];
+    for (int i = 0; i < 4; ++i) {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        s[i] = z ^ (z >> 31);
+    }
+    auto rotl = [](uint64_t v, int k) { return (v << k) | (v >> (64 - k)); };
+    for (size_t pos = 0; pos < n; pos += 8) {
+        const uint64_t r = rotl(s[1] * 5, 7) * 9;
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        const size_t m = n - pos < 8 ? n - pos : 8;
+        for (size_t b = 0; b < m; ++b) out[pos + b] = (uint8_t)(r >> (8 * b));
+    }
+    return 0;
+}
+
 const char *dil_error_string(int code) {
     switch (code) {
         case DIL_OK: return "success";

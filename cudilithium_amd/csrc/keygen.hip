@@ -1,0 +1,219 @@
+// Batched key generation (reference: gpu_keypair, src/keypair.cu:35-311, one
+// 32-lane warp per key with ExpandA streamed through shared memory).
+//
+// Four launches, each with the lane mapping its work wants:
+//   kg_seed   lane per key     seed (explicit or SHAKE128(le64(ctr))), then
+//                              rho | rhoprime | key = SHAKE256(seed)
+//   kg_eta    lane per s-poly  ExpandS into int8 scratch, pack eta into sk
+//   kg_main   G keys per workgroup: NTT(s1) into LDS, then one lane per
+//             ExpandA entry (G*K*L sponges) folds each accepted coefficient
+//             straight into t^[k][ctr] += mont(a, s1^[l][ctr]) with LDS
+//             atomics -- A is never stored; the sum is exact int32 so the
+//             accumulation order does not matter.  Then INTT, + s2, caddq,
+//             Power2Round and packing of t1 / t0, one wavefront per row.
+//   kg_tr     lane per key     tr = SHAKE256(pk)[0:32]
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+#include "layout.hpp"
+#include "pipelines.hpp"
+#include "poly.hpp"
+#include "sample.hpp"
+
+namespace dil {
+
+__device__ __forceinline__ void store_le64(uint8_t *p, uint64_t v) {
+    if (((uintptr_t)p & 7) == 0) {
+        *reinterpret_cast<uint64_t *>(p) = v;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) p[i] = (uint8_t)(v >> (8 * i));
+    }
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void kg_seed_kernel(KeygenArgs a) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    uint64_t seed[4];
+    Lane64 s[25];
+    if (a.seeds) {
+        const uint8_t *sp = a.seeds + i * a.seed_pitch;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) seed[w] = load_le64(sp + 8 * w);
+    } else {
+        // keypair.cu:50-60: seed = SHAKE128(le64(rand_index + 2i + 1))[0:32]
+        const uint64_t in[1] = {(uint64_t)a.rand_index + 2 * (uint64_t)i + 1};
+        shake_absorb_regs<SHAKE128_RATE, 8>(s, in);
+        squeeze_words<4>(s, seed);
+    }
+    uint64_t out[16];
+    shake_absorb_regs<SHAKE256_RATE, SEEDBYTES>(s, seed);
+    squeeze_words<16>(s, out);
+    uint8_t *pk = a.pk + i * a.pk_pitch;
+    uint8_t *sk = a.sk + i * a.sk_pitch;
+    uint8_t *rp = a.scratch + i * a.scratch_pitch + KeypairLayout<P>::RHOPRIME - KeypairLayout<P>::SCRATCH;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        store_le64(pk + 8 * w, out[w]);
+        store_le64(sk + 8 * w, out[w]);
+        store_le64(sk + SEEDBYTES + 8 * w, out[12 + w]);
+    }
+#pragma unroll
+    for (int w = 0; w < 8; ++w) store_le64(rp + 8 * w, out[4 + w]);
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void kg_eta_kernel(KeygenArgs a) {
+    using Z = Sizes<P>;
+    constexpr int NP = P::L + P::K;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n * NP) return;
+    const size_t i = t / NP;
+    const int p = (int)(t % NP);
+    uint8_t *scr = a.scratch + i * a.scratch_pitch;
+    const uint8_t *rp = scr + KeypairLayout<P>::RHOPRIME - KeypairLayout<P>::SCRATCH;
+    int8_t *s8 = reinterpret_cast<int8_t *>(scr + KeypairLayout<P>::S8 - KeypairLayout<P>::SCRATCH) + p * N;
+    uint64_t seed[8];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) seed[w] = *reinterpret_cast<const uint64_t *>(rp + 8 * w);
+    // nonce l for s1[l], L + k for s2[k] (keypair.cu:84-89, 202-207)
+    eta_poly<P::ETA>(seed, (uint32_t)p, [&](int ctr, int32_t v) { s8[ctr] = (int8_t)v; });
+    // pack eta - s (keypair.cu:116-130)
+    uint8_t *dst = a.sk + i * a.sk_pitch + Z::SK_S1 + p * Z::POLYETA_BYTES;
+    for (int g = 0; g < N / 8; ++g) {
+        uint32_t c[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) c[j] = (uint32_t)(P::ETA - s8[8 * g + j]);
+        pack8_store<Z::ETA_BITS>(dst + g * Z::ETA_BITS, c);
+    }
+}
+
+template <class P>
+struct KgMainCfg {
+    static constexpr int G = P::MODE == 2 ? 16 : (P::MODE == 3 ? 8 : 4);
+};
+
+template <class P>
+struct KgShared {
+    static constexpr int G = KgMainCfg<P>::G;
+    ZetaLds zt;
+    int32_t x[4][NTT_XBUF];
+    int32_t shat[G][P::L][N];
+    int32_t acc[G][P::K][N];
+};
+
+template <class P>
+__global__ __launch_bounds__(256) void kg_main_kernel(KeygenArgs a) {
+    using Z = Sizes<P>;
+    using KL = KeypairLayout<P>;
+    constexpr int G = KgMainCfg<P>::G;
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    KgShared<P> &sh = *reinterpret_cast<KgShared<P> *>(smem_raw);
+    const size_t item0 = (size_t)blockIdx.x * G;
+    const int wave = threadIdx.x >> 6;
+    zeta_lds_load(sh.zt);
+    for (int j = threadIdx.x; j < G * P::K * N; j += blockDim.x) (&sh.acc[0][0][0])[j] = 0;
+    __syncthreads();
+
+    // NTT(s1) for the G keys
+    for (int t = wave; t < G * P::L; t += 4) {
+        const int g = t / P::L, l = t % P::L;
+        if (item0 + g >= a.n) continue;
+        const int8_t *s8 = reinterpret_cast<const int8_t *>(a.scratch + (item0 + g) * a.scratch_pitch + KL::S8 -
+                                                            KL::SCRATCH);
+        int32_t v[4];
+        wave_load_i8_A(v, s8 + l * N);
+        ntt_wave(v, sh.x[wave], sh.zt);
+        wave_store_D(sh.shat[g][l], v);
+    }
+    __syncthreads();
+
+    // ExpandA: one sponge per (key, k, l), accepted coefficients folded into t^
+    for (int t = threadIdx.x; t < G * P::K * P::L; t += blockDim.x) {
+        const int g = t / (P::K * P::L), kl = t % (P::K * P::L);
+        const int k = kl / P::L, l = kl % P::L;
+        if (item0 + g >= a.n) continue;
+        const uint8_t *pk = a.pk + (item0 + g) * a.pk_pitch;
+        uint64_t rho[4];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) rho[w] = load_le64(pk + 8 * w);
+        int32_t *acc = sh.acc[g][k];
+        const int32_t *s1 = sh.shat[g][l];
+        uniform_poly(rho, (uint32_t)((k << 8) | l), [&](int ctr, int32_t v) { atomicAdd(&acc[ctr], mont_mul(v, s1[ctr])); });
+    }
+    __syncthreads();
+
+    // t = INTT(reduce32(t^)) + s2, caddq, Power2Round, pack (keypair.cu:251-301)
+    for (int t = wave; t < G * P::K; t += 4) {
+        const int g = t / P::K, k = t % P::K;
+        const size_t item = item0 + g;
+        if (item >= a.n) continue;
+        int32_t v[4];
+        wave_load_D(v, sh.acc[g][k]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = reduce32(v[r]);
+        intt_wave(v, sh.x[wave], sh.zt);
+        const int8_t *s2 = reinterpret_cast<const int8_t *>(a.scratch + item * a.scratch_pitch + KL::S8 - KL::SCRATCH) +
+                           (P::L + k) * N;
+        int32_t s2v[4], t1[4], t0[4];
+        wave_load_i8_A(s2v, s2);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) t1[r] = power2round(caddq(v[r] + s2v[r]), t0[r]);
+        layout_A_to_D(t1, sh.x[wave]);
+        layout_A_to_D(t0, sh.x[wave]);
+        uint32_t f1[4], f0[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            f1[r] = (uint32_t)t1[r];
+            f0[r] = (uint32_t)((1 << (D - 1)) - t0[r]);
+        }
+        wave_pack_D<10>(a.pk + item * a.pk_pitch + SEEDBYTES + k * POLYT1_BYTES, f1);
+        wave_pack_D<13>(a.sk + item * a.sk_pitch + Z::SK_T0 + k * POLYT0_BYTES, f0);
+    }
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void kg_tr_kernel(KeygenArgs a) {
+    using Z = Sizes<P>;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const uint8_t *pk = a.pk + i * a.pk_pitch;
+    Lane64 s[25];
+    shake_absorb<SHAKE256_RATE>(s, [&](uint32_t w, uint32_t nb) {
+        return nb == 8 ? load_le64(pk + 8 * w) : load_le_partial(pk + 8 * w, nb);
+    }, (uint32_t)Z::PK_BYTES);
+    uint64_t tr[4];
+    squeeze_words<4>(s, tr);
+    uint8_t *sk = a.sk + i * a.sk_pitch + Z::SK_TR;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) store_le64(sk + 8 * w, tr[w]);
+}
+
+template <class P>
+static int keygen_impl(const KeygenArgs &a, hipStream_t st) {
+    if (a.n == 0) return 0;
+    constexpr int NP = P::L + P::K;
+    constexpr int G = KgMainCfg<P>::G;
+    const unsigned b1 = (unsigned)((a.n + 255) / 256);
+    hipLaunchKernelGGL(kg_seed_kernel<P>, dim3(b1), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(kg_eta_kernel<P>, dim3((unsigned)((a.n * NP + 255) / 256)), dim3(256), 0, st, a);
+    const size_t smem = sizeof(KgShared<P>);
+    hipError_t e = hipFuncSetAttribute((const void *)kg_main_kernel<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)smem);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(kg_main_kernel<P>, dim3((unsigned)((a.n + G - 1) / G)), dim3(256), smem, st, a);
+    hipLaunchKernelGGL(kg_tr_kernel<P>, dim3(b1), dim3(256), 0, st, a);
+    return (int)hipGetLastError();
+}
+
+int launch_keygen(int mode, const KeygenArgs &a, hipStream_t st) {
+    switch (mode) {
+        case 2: return keygen_impl<Params<2>>(a, st);
+        case 3: return keygen_impl<Params<3>>(a, st);
+        case 5: return keygen_impl<Params<5>>(a, st);
+        default: return -1;
+    }
+}
+
+}  // namespace dil

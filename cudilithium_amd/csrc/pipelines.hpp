@@ -1,0 +1,72 @@
+// Argument blocks and launchers of the keygen / sign / verify pipelines.
+// Every array is addressed as base + item * pitch so the same kernels serve
+// the drop-in pitched pools and the device-resident batch API.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace dil {
+
+struct KeygenArgs {
+    uint8_t *pk;
+    size_t pk_pitch;
+    uint8_t *sk;
+    size_t sk_pitch;
+    uint8_t *scratch;  // KeypairLayout::SCRATCH area of each row
+    size_t scratch_pitch;
+    const uint8_t *seeds;  // 32 bytes per key, or null for counter seeds
+    size_t seed_pitch;
+    uint64_t rand_index;
+    size_t n;
+};
+
+// Messages: item i is msg + (moff ? moff[i] : i * mpitch), length
+// (mlen ? mlen[i] : mlen_const).
+struct MsgRef {
+    const uint8_t *msg;
+    const uint64_t *moff;
+    size_t mpitch;
+    const uint32_t *mlen;
+    uint32_t mlen_const;
+    __host__ __device__ const uint8_t *ptr(size_t i) const { return msg + (moff ? moff[i] : i * mpitch); }
+    __host__ __device__ uint32_t len(size_t i) const { return mlen ? mlen[i] : mlen_const; }
+};
+
+struct VerifyArgs {
+    const uint8_t *sig;
+    size_t sig_pitch;
+    const uint8_t *pk;
+    size_t pk_pitch;
+    MsgRef m;
+    uint8_t *work;  // VerifyLayout rows (only HASH.. fields used)
+    size_t work_pitch;
+    int32_t *ret;
+    size_t ret_pitch;  // in int32 units
+    size_t n;
+};
+
+// Host-side per-item state of the per-stage scheduler, laid out as the
+// reference's sign_lut_element (api.cuh:31-34).
+struct SignLutEntry {
+    int done_flag;
+    uint16_t nonce;
+};
+
+struct SignArgs;
+struct SlotRef;
+
+int launch_keygen(int mode, const KeygenArgs &a, hipStream_t st);
+int launch_verify(int mode, const VerifyArgs &a, hipStream_t st);
+int launch_sign_fixed(int mode, const SignArgs &a, hipStream_t st);
+int launch_sign_round(int mode, const SignArgs &a, const SlotRef &sr, const uint32_t *d_exec, size_t nslot,
+                      int by_item, uint8_t *d_done, hipStream_t st);
+int launch_sign_copy(int mode, const SignArgs &a, const SlotRef &sr, const uint64_t *d_copy, size_t ncopy,
+                     hipStream_t st);
+int sign_perstage(int mode, const SignArgs &a, const SlotRef &item_slots, const SlotRef &temp_slots, size_t thr,
+                  SignLutEntry *sign_lut, uint32_t *h_exec, uint32_t *d_exec, uint8_t *h_done, uint8_t *d_done,
+                  uint64_t *h_copy, uint64_t *d_copy, hipStream_t st);
+
+}  // namespace dil

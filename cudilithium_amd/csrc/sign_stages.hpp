@@ -1,0 +1,302 @@
+// Device-side stages of deterministic signing, shared by the per-stage
+// (one launch per stage, host-driven rejection rounds) form and the
+// persistent work-queue engine.
+//
+// Fixed part per item (api.cu:117-182):
+//   mu = SHAKE256(tr || m), rhoprime = SHAKE256(key || mu)   lane per item
+//   A = ExpandA(rho)                                         lane per entry
+//   s1^, s2^, t0^ = NTT(unpack(sk))                          wave per poly
+// One attempt (item, nonce) in a "slot" (sign.cu:463-1196):
+//   y    lane per (slot, l)   raw ExpandMask squeeze of rhoprime || L*nonce+l
+//   w    wave per slot        NTT(y), A y^, INTT, caddq -> w; pack w1
+//   c    lane per slot        c~ = SHAKE256(mu || w1), c = SampleInBall(c~)
+//   z    wave per slot        z = y + INTT(c^ s1^) and the three norm checks,
+//                             MakeHint, pack z and h
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "layout.hpp"
+#include "pipelines.hpp"
+#include "poly.hpp"
+#include "sample.hpp"
+
+namespace dil {
+
+struct SignArgs {
+    uint8_t *sig;  // final signatures
+    size_t sig_pitch;
+    const uint8_t *sk;
+    size_t sk_pitch;
+    MsgRef m;
+    uint8_t *rows;  // SignLayout rows (item state)
+    size_t row_pitch;
+    size_t off_ints;  // byte offset of the int32 arrays (A, s1^, s2^, t0^) in a row
+    size_t n;
+};
+
+// Scratch and signature destination of attempt slots.
+struct SlotRef {
+    uint8_t *base;  // SlotLayout of slot s at base + s * pitch
+    size_t pitch;
+    uint8_t *sig;  // signature bytes of slot s at sig + s * sig_pitch
+    size_t sig_pitch;
+};
+
+template <class P>
+struct ItemView {
+    using Z = Sizes<P>;
+    using SLy = SignLayout<P>;
+    uint8_t *row;
+    const int32_t *mat, *s1h, *s2h, *t0h;
+    __device__ ItemView(const SignArgs &a, size_t item) {
+        row = a.rows + item * a.row_pitch;
+        mat = reinterpret_cast<const int32_t *>(row + a.off_ints);
+        s1h = mat + P::K * P::L * N;
+        s2h = s1h + P::L * N;
+        t0h = s2h + P::K * N;
+    }
+};
+
+__device__ __forceinline__ void load_words8(uint64_t (&w)[8], const uint8_t *p) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[i] = reinterpret_cast<const uint64_t *>(p)[i];
+}
+
+// ---------------------------------------------------------------- fixed part
+
+template <class P>
+__device__ __forceinline__ void stage_mu(const SignArgs &a, size_t i) {
+    using Z = Sizes<P>;
+    using SLy = SignLayout<P>;
+    const uint8_t *sk = a.sk + i * a.sk_pitch;
+    uint64_t tr[4], key[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        tr[w] = load_le64(sk + Z::SK_TR + 8 * w);
+        key[w] = load_le64(sk + Z::SK_KEY + 8 * w);
+    }
+    const uint8_t *m = a.m.ptr(i);
+    const uint32_t mlen = a.m.len(i);
+    Lane64 s[25];
+    shake_absorb<SHAKE256_RATE>(s, [&](uint32_t w, uint32_t nb) -> uint64_t {
+        if (w < 4) return w == 0 ? tr[0] : (w == 1 ? tr[1] : (w == 2 ? tr[2] : tr[3]));
+        const uint8_t *p = m + 8 * (w - 4);
+        return nb == 8 ? load_le64(p) : load_le_partial(p, nb);
+    }, SEEDBYTES + mlen);
+    uint64_t in[12];
+    uint64_t mu[8];
+    squeeze_words<8>(s, mu);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) in[w] = key[w];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) in[4 + w] = mu[w];
+    shake_absorb_regs<SHAKE256_RATE, SEEDBYTES + CRHBYTES>(s, in);
+    uint64_t rp[8];
+    squeeze_words<8>(s, rp);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(a.rows + i * a.row_pitch + SLy::MU);
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+        dst[w] = mu[w];
+        dst[8 + w] = rp[w];
+    }
+}
+
+template <class P>
+__device__ __forceinline__ void stage_expand_a(const SignArgs &a, size_t i, int k, int l) {
+    const uint8_t *sk = a.sk + i * a.sk_pitch;
+    uint64_t rho[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) rho[w] = load_le64(sk + 8 * w);
+    int32_t *dst = reinterpret_cast<int32_t *>(a.rows + i * a.row_pitch + a.off_ints) + (k * P::L + l) * N;
+    uniform_poly(rho, (uint32_t)((k << 8) | l), [&](int ctr, int32_t v) { dst[ctr] = v; });
+}
+
+// poly p of the item: s1[0..L), s2[0..K), t0[0..K) -> NTT domain (layout D)
+template <class P>
+__device__ __forceinline__ void stage_prep(const SignArgs &a, size_t i, int p, int32_t *x, const ZetaLds &zt) {
+    using Z = Sizes<P>;
+    const uint8_t *sk = a.sk + i * a.sk_pitch;
+    int32_t *dst = reinterpret_cast<int32_t *>(a.rows + i * a.row_pitch + a.off_ints) + (P::K * P::L + p) * N;
+    int32_t v[4];
+    uint32_t f[4];
+    if (p < P::L + P::K) {
+        wave_fields_A<Z::ETA_BITS>(f, sk + Z::SK_S1 + p * Z::POLYETA_BYTES);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = P::ETA - (int32_t)f[r];
+    } else {
+        wave_fields_A<13>(f, sk + Z::SK_T0 + (p - P::L - P::K) * POLYT0_BYTES);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = (1 << (D - 1)) - (int32_t)f[r];
+    }
+    ntt_wave(v, x, zt);
+    wave_store_D(dst, v);
+}
+
+// ------------------------------------------------------------- attempt part
+
+template <class P>
+__device__ __forceinline__ void stage_y(const SignArgs &a, size_t item, uint32_t nonce, int l, uint8_t *slot) {
+    using Z = Sizes<P>;
+    uint64_t rp[8];
+    load_words8(rp, a.rows + item * a.row_pitch + SignLayout<P>::RHOPRIME);
+    mask_raw(rp, (uint32_t)(P::L * nonce + l) & 0xFFFF,
+             reinterpret_cast<uint64_t *>(slot + SlotLayout<P>::YRAW + l * Z::YRAW_BYTES));
+}
+
+template <class P>
+__device__ __forceinline__ void load_y_A(int32_t (&v)[4], const uint8_t *slot, int l) {
+    using Z = Sizes<P>;
+    uint32_t f[4];
+    wave_fields_A<Z::Z_BITS>(f, slot + SlotLayout<P>::YRAW + l * Z::YRAW_BYTES);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = P::GAMMA1 - (int32_t)f[r];
+}
+
+template <class P>
+__device__ __forceinline__ void stage_w(const SignArgs &a, size_t item, uint8_t *slot, int32_t *x, const ZetaLds &zt) {
+    using Z = Sizes<P>;
+    using SL = SlotLayout<P>;
+    const ItemView<P> it(a, item);
+    int32_t yh[P::L][4];
+#pragma unroll
+    for (int l = 0; l < P::L; ++l) {
+        load_y_A<P>(yh[l], slot, l);
+        ntt_wave(yh[l], x, zt);
+    }
+    for (int k = 0; k < P::K; ++k) {
+        int32_t acc[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int l = 0; l < P::L; ++l) {
+            int32_t av[4];
+            wave_load_D(av, it.mat + (k * P::L + l) * N);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] += mont_mul(av[r], yh[l][r]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = reduce32(acc[r]);
+        intt_wave(acc, x, zt);
+        int32_t w1[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            acc[r] = caddq(acc[r]);
+            int32_t w0;
+            w1[r] = decompose<P>(acc[r], w0);
+        }
+        wave_store_A(reinterpret_cast<int32_t *>(slot + SL::W) + k * N, acc);
+        layout_A_to_D(w1, x);
+        uint32_t u[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) u[r] = (uint32_t)w1[r];
+        wave_pack_D<Z::W1_BITS>(slot + SL::W1PK + k * Z::POLYW1_BYTES, u);
+    }
+}
+
+template <class P>
+__device__ __forceinline__ void stage_c(const SignArgs &a, size_t item, uint8_t *slot, uint8_t *sig_out) {
+    using Z = Sizes<P>;
+    using SL = SlotLayout<P>;
+    const uint64_t *mu = reinterpret_cast<const uint64_t *>(a.rows + item * a.row_pitch + SignLayout<P>::MU);
+    const uint64_t *w1 = reinterpret_cast<const uint64_t *>(slot + SL::W1PK);
+    Lane64 s[25];
+    shake_absorb<SHAKE256_RATE>(s, [&](uint32_t w, uint32_t) { return w < 8 ? mu[w] : w1[w - 8]; },
+                                (uint32_t)Z::CHASH_IN_BYTES);
+    uint64_t ct[4];
+    squeeze_words<4>(s, ct);
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) sig_out[8 * w + b] = (uint8_t)(ct[w] >> (8 * b));
+    challenge<P::TAU>(ct, reinterpret_cast<int8_t *>(slot + SL::CP));
+}
+
+// Returns true (wave-uniform) when the attempt passes every check; then
+// sig_out holds the complete signature.
+template <class P>
+__device__ __forceinline__ bool stage_z(const SignArgs &a, size_t item, uint8_t *slot, uint8_t *sig_out, int32_t *x,
+                                        const ZetaLds &zt) {
+    using Z = Sizes<P>;
+    using SL = SlotLayout<P>;
+    const ItemView<P> it(a, item);
+    const int lane = lane_id();
+    int32_t ch[4];
+    wave_load_i8_A(ch, reinterpret_cast<const int8_t *>(slot + SL::CP));
+    ntt_wave(ch, x, zt);
+
+    int32_t z[P::L][4];
+#pragma unroll
+    for (int l = 0; l < P::L; ++l) {
+        int32_t s[4], y[4];
+        wave_load_D(s, it.s1h + l * N);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) z[l][r] = mont_mul(ch[r], s[r]);
+        intt_wave(z[l], x, zt);
+        load_y_A<P>(y, slot, l);
+        bool big = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            z[l][r] = reduce32(z[l][r] + y[r]);
+            big |= exceeds(z[l][r], P::GAMMA1 - P::BETA);
+        }
+        if (wave_any(big)) return false;
+    }
+
+    uint8_t *h = sig_out + Z::SIG_H;
+    int pos = 0;
+    for (int k = 0; k < P::K; ++k) {
+        int32_t r0[4], ct0[4], w[4];
+        wave_load_D(r0, it.s2h + k * N);
+        wave_load_D(ct0, it.t0h + k * N);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            r0[r] = mont_mul(ch[r], r0[r]);
+            ct0[r] = mont_mul(ch[r], ct0[r]);
+        }
+        intt_wave(r0, x, zt);
+        wave_load_A(w, reinterpret_cast<const int32_t *>(slot + SL::W) + k * N);
+        int32_t w1[4];
+        bool big = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int32_t w0;
+            w1[r] = decompose<P>(w[r], w0);
+            r0[r] = reduce32(w0 - r0[r]);
+            big |= exceeds(r0[r], P::GAMMA2 - P::BETA);
+        }
+        if (wave_any(big)) return false;
+        intt_wave(ct0, x, zt);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            ct0[r] = reduce32(ct0[r]);
+            big |= exceeds(ct0[r], P::GAMMA2);
+        }
+        if (wave_any(big)) return false;
+        // MakeHint in ascending coefficient order j = 64 r + lane
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int hb = make_hint<P>(r0[r] + ct0[r], w1[r]);
+            const uint64_t m = __ballot(hb);
+            const int cnt = __popcll(m);
+            if (pos + cnt > P::OMEGA) return false;
+            if (hb) {
+                const int idx = pos + __popcll(m & ((1ull << lane) - 1ull));
+                h[idx] = (uint8_t)(64 * r + lane);
+            }
+            pos += cnt;
+        }
+        if (lane == 0) h[P::OMEGA + k] = (uint8_t)pos;
+    }
+    for (int j = pos + lane; j < P::OMEGA; j += 64) h[j] = 0;
+
+#pragma unroll
+    for (int l = 0; l < P::L; ++l) {
+        layout_A_to_D(z[l], x);
+        uint32_t f[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) f[r] = (uint32_t)(P::GAMMA1 - z[l][r]);
+        wave_pack_D<Z::Z_BITS>(sig_out + Z::SIG_Z + l * Z::POLYZ_BYTES, f);
+    }
+    return true;
+}
+
+}  // namespace dil

@@ -47,6 +47,147 @@ size_t dil_secret_key_bytes(int mode);
 size_t dil_signature_bytes(int mode);
 
 /* ---------------------------------------------------------------------- */
+/* Drop-in batched API (include/api.cuh:27-63)                            */
+/* ---------------------------------------------------------------------- */
+
+/* sign_lut_element (api.cuh:31-34) */
+typedef struct {
+    int done_flag;
+    uint16_t nonce;
+} dil_sign_lut_element;
+
+/* copy_lut_element (api.cuh:36-39) */
+typedef struct {
+    uint64_t exec_idx;
+    uint64_t sign_idx;
+} dil_copy_lut_element;
+
+/* task_lut (api.cuh:41-49).  The reference keeps sign_lut in a std::vector;
+ * here the caller passes an array of at least `batch` elements.  All other
+ * members are allocated by the caller exactly as in the reference
+ * (tests/test_vectors.cu:104-113): exec_threshold entries each, h_* pinned
+ * host memory, d_* device memory. */
+typedef struct {
+    dil_sign_lut_element *sign_lut;
+    size_t sign_lut_len;
+    uint32_t *h_exec_lut;
+    uint8_t *h_done_lut;
+    dil_copy_lut_element *h_copy_lut;
+    uint32_t *d_exec_lut;
+    uint8_t *d_done_lut;
+    dil_copy_lut_element *d_copy_lut;
+} dil_task_lut;
+
+#define DIL_POOL_KEYPAIR 0
+#define DIL_POOL_SIGN 1
+#define DIL_POOL_SIGN_TEMP 2
+#define DIL_POOL_VERIFY 3
+
+#define DIL_ENGINE_PERSISTENT 0 /* device work queue, one persistent launch */
+#define DIL_ENGINE_PERSTAGE 1   /* one launch per stage, host-driven rounds */
+
+/* Minimum pitch of each pool row this library needs.  Each is at most the
+ * row size the reference's callers compute (tests/test_vectors.cu:64-70,
+ * 81-100, 117-128, 149-166), which dil_reference_pool_row_bytes returns, so
+ * pools sized by the reference's formulas work unchanged. */
+size_t dil_keypair_pool_row_bytes(int mode);
+size_t dil_sign_pool_row_bytes(int mode, size_t mlen);
+size_t dil_sign_temp_row_bytes(int mode);
+size_t dil_verify_pool_row_bytes(int mode, size_t mlen);
+size_t dil_reference_pool_row_bytes(int mode, int which, size_t mlen);
+
+/* crypto_sign_keypair (api.cuh:27-29, api.cu:26-61): batch keys whose seeds
+ * are SHAKE128(le64(rand_index + 2i + 1))[0:32]; pk/sk are host buffers
+ * (packed, stride CRYPTO_PUBLICKEYBYTES / CRYPTO_SECRETKEYBYTES), copied back
+ * asynchronously on `stream` (the caller synchronises). */
+int dil_crypto_sign_keypair(int mode, uint8_t *pk, uint8_t *sk, uint8_t *d_keypair_mem_pool,
+                            size_t keypair_mem_pool_pitch, size_t batch_size, void *stream,
+                            size_t rand_index);
+
+/* crypto_sign_signature (api.cuh:51-56, api.cu:63-452): deterministic
+ * signatures of batch messages of mlen bytes (host, stride m_pitch) under
+ * host secret keys; signatures to host `sig` with stride sig_pitch;
+ * *siglen = CRYPTO_BYTES.  Per item the result is the attempt with the first
+ * successful nonce, as in the reference.  This entry point runs the
+ * persistent work-queue engine; exec_threshold is the number of attempt
+ * slots (temp pool rows).  batch_size <= 65536. */
+int dil_crypto_sign_signature(int mode, uint8_t *sig, size_t sig_pitch, size_t *siglen,
+                              const uint8_t *m, size_t m_pitch, size_t mlen, const uint8_t *sk,
+                              uint8_t *d_sign_mem_pool, size_t d_sign_mem_pool_pitch,
+                              uint8_t *d_temp_mem_pool, size_t d_temp_mem_pool_pitch,
+                              dil_task_lut *lut, size_t exec_threshold, size_t batch_size,
+                              void *stream);
+/* Same contract, run as the reference architecture: one launch per stage and
+ * host-driven rejection rounds of exec_threshold attempts. */
+int dil_crypto_sign_signature_perstage(int mode, uint8_t *sig, size_t sig_pitch, size_t *siglen,
+                                       const uint8_t *m, size_t m_pitch, size_t mlen,
+                                       const uint8_t *sk, uint8_t *d_sign_mem_pool,
+                                       size_t d_sign_mem_pool_pitch, uint8_t *d_temp_mem_pool,
+                                       size_t d_temp_mem_pool_pitch, dil_task_lut *lut,
+                                       size_t exec_threshold, size_t batch_size, void *stream);
+
+/* crypto_sign_verify (api.cuh:58-63, api.cu:454-515): ret[i] = 0 for a valid
+ * signature, -1 otherwise; siglen != CRYPTO_BYTES sets every ret to -1
+ * without GPU work.  ret is copied back asynchronously on `stream`. */
+int dil_crypto_sign_verify(int mode, int *ret, const uint8_t *sig, size_t sig_pitch, size_t siglen,
+                           const uint8_t *m, size_t m_pitch, size_t mlen, const uint8_t *pk,
+                           uint8_t *d_verify_mem_pool, size_t verify_mem_pool_pitch,
+                           size_t batch_size, void *stream);
+
+/* Per-mode names with the reference's exact parameter lists (one library per
+ * mode in the reference, src/CMakeLists.txt:3-16). */
+int dil2_crypto_sign_keypair(uint8_t *pk, uint8_t *sk, uint8_t *d_pool, size_t pitch, size_t batch,
+                             void *stream, size_t rand_index);
+int dil3_crypto_sign_keypair(uint8_t *pk, uint8_t *sk, uint8_t *d_pool, size_t pitch, size_t batch,
+                             void *stream, size_t rand_index);
+int dil5_crypto_sign_keypair(uint8_t *pk, uint8_t *sk, uint8_t *d_pool, size_t pitch, size_t batch,
+                             void *stream, size_t rand_index);
+int dil2_crypto_sign_signature(uint8_t *sig, size_t sig_pitch, size_t *siglen, const uint8_t *m,
+                               size_t m_pitch, size_t mlen, const uint8_t *sk, uint8_t *d_sign_pool,
+                               size_t sign_pitch, uint8_t *d_temp_pool, size_t temp_pitch,
+                               dil_task_lut *lut, size_t exec_threshold, size_t batch, void *stream);
+int dil3_crypto_sign_signature(uint8_t *sig, size_t sig_pitch, size_t *siglen, const uint8_t *m,
+                               size_t m_pitch, size_t mlen, const uint8_t *sk, uint8_t *d_sign_pool,
+                               size_t sign_pitch, uint8_t *d_temp_pool, size_t temp_pitch,
+                               dil_task_lut *lut, size_t exec_threshold, size_t batch, void *stream);
+int dil5_crypto_sign_signature(uint8_t *sig, size_t sig_pitch, size_t *siglen, const uint8_t *m,
+                               size_t m_pitch, size_t mlen, const uint8_t *sk, uint8_t *d_sign_pool,
+                               size_t sign_pitch, uint8_t *d_temp_pool, size_t temp_pitch,
+                               dil_task_lut *lut, size_t exec_threshold, size_t batch, void *stream);
+int dil2_crypto_sign_verify(int *ret, const uint8_t *sig, size_t sig_pitch, size_t siglen,
+                            const uint8_t *m, size_t m_pitch, size_t mlen, const uint8_t *pk,
+                            uint8_t *d_pool, size_t pitch, size_t batch, void *stream);
+int dil3_crypto_sign_verify(int *ret, const uint8_t *sig, size_t sig_pitch, size_t siglen,
+                            const uint8_t *m, size_t m_pitch, size_t mlen, const uint8_t *pk,
+                            uint8_t *d_pool, size_t pitch, size_t batch, void *stream);
+int dil5_crypto_sign_verify(int *ret, const uint8_t *sig, size_t sig_pitch, size_t siglen,
+                            const uint8_t *m, size_t m_pitch, size_t mlen, const uint8_t *pk,
+                            uint8_t *d_pool, size_t pitch, size_t batch, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident batches (extensions; all pointers are device memory)   */
+/* Item i of an array lives at base + i * pitch.  Messages: item i is      */
+/* d_msg + d_moff[i], d_mlen[i] bytes (per-item lengths).                 */
+/* ---------------------------------------------------------------------- */
+
+/* Keygen from explicit 32-byte seeds (d_seeds != NULL), or from the
+ * reference's counter seeds (d_seeds == NULL, rand_index as above).
+ * d_work: n rows of at least dil_keypair_work_row_bytes(mode), pitch a
+ * multiple of 8. */
+size_t dil_keypair_work_row_bytes(int mode);
+int dil_keypair_device(int mode, uint8_t *d_pk, size_t pk_pitch, uint8_t *d_sk, size_t sk_pitch,
+                       const uint8_t *d_seeds, size_t seed_pitch, uint64_t rand_index, size_t n,
+                       uint8_t *d_work, size_t work_pitch, void *stream);
+
+/* Verify n (sig, msg, pk) triples; d_ret[i] = 0 or -1.  d_work: n rows of at
+ * least dil_verify_work_row_bytes(mode), pitch a multiple of 256. */
+size_t dil_verify_work_row_bytes(int mode);
+int dil_verify_device(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch,
+                      const uint8_t *d_pk, size_t pk_pitch, const uint8_t *d_msg,
+                      const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, uint8_t *d_work,
+                      size_t work_pitch, void *stream);
+
+/* ---------------------------------------------------------------------- */
 /* Standalone NTT (include/ntt.cuh:30-62, src/ntt.cu:77-207)              */
 /* Polynomials are 256 int32 coefficients; consecutive polynomials start  */
 /* pitch_words int32 apart (pitch_words >= 256).  Device pointers.        */
@@ -93,6 +234,10 @@ int dil_event_record(void *event, void *stream);
 int dil_event_synchronize(void *event);
 int dil_event_elapsed_ms(float *ms, void *start, void *stop);
 const char *dil_error_string(int code);
+
+/* Synthetic workload bytes: the little-endian outputs of xoshiro256**
+ * (Blackman-Vigna) seeded through splitmix64 from `seed`. */
+int dil_xoshiro_bytes(uint64_t seed, uint8_t *out, size_t n);
 
 #ifdef __cplusplus
 }

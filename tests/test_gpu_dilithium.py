@@ -1,0 +1,146 @@
+"""GPU parity of keygen / sign / verify against the reference's KAT pins and
+the oracle, through the C-ABI (drop-in and device-resident entry points)."""
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import kat
+
+pytestmark = pytest.mark.gpu
+
+
+def _engines():
+    from cudilithium_amd.api import ENGINE_PERSISTENT, ENGINE_PERSTAGE
+    return [ENGINE_PERSISTENT, ENGINE_PERSTAGE]
+
+
+@pytest.mark.parametrize("mode", [2, 3, 5])
+def test_gpu_reproduces_reference_tvecs(mode):
+    """tests/test_vectors.cu on the GPU: 10,000 counter-seeded keys, 32-byte
+    messages, exec_threshold 2048, verify; SHA-256 of the tvecs file must be
+    the reference's pin (tests/CMakeLists.txt:21-23)."""
+    from cudilithium_amd.api import Dilithium
+    d = Dilithium(mode)
+    m = kat.messages()
+    pk, sk = d.keypair(kat.NVECTORS, rand_index=0)
+    sig = d.sign(m, sk, exec_threshold=2048)
+    ret = d.verify(sig, m, pk)
+    assert (ret == 0).all()
+    assert kat.tvecs_sha256(m, pk, sk, sig, ret) == kat.PINS[mode]
+
+
+@pytest.mark.parametrize("mode", [2, 3, 5])
+def test_perstage_engine_matches_oracle(mode):
+    from cudilithium_amd.api import ENGINE_PERSTAGE, Dilithium
+    d = Dilithium(mode)
+    n = 300
+    pk, sk = oracle.keypair_counter_batch(mode, n, rand_index=77)
+    m = np.random.default_rng(mode).integers(0, 256, (n, 48), dtype=np.uint8)
+    exp, _ = oracle.sign_batch(mode, m, sk)
+    # exec_threshold 64 < batch exercises full rounds then tail rounds
+    got = d.sign(m, sk, exec_threshold=64, engine=ENGINE_PERSTAGE)
+    assert np.array_equal(got, exp)
+
+
+def test_config1_dil2_batch128_explicit_seeds():
+    """Dilithium2, batch 128, 32-byte seeds and 64-byte messages from the
+    seeded xoshiro256** stream: GPU bytes equal the oracle's, verdicts 0."""
+    from cudilithium_amd.api import Dilithium, keypair_device
+    from cudilithium_amd.workloads import config1
+    seeds, m = config1()
+    pk, sk = keypair_device(2, seeds=seeds)
+    epk, esk = oracle.keypair_batch(2, seeds)
+    assert np.array_equal(pk, epk) and np.array_equal(sk, esk)
+    d = Dilithium(2)
+    sig = d.sign(m, sk)
+    esig, _ = oracle.sign_batch(2, m, sk)
+    assert np.array_equal(sig, esig)
+    assert (d.verify(sig, m, pk) == 0).all()
+
+
+@pytest.mark.parametrize("mode", [2, 3, 5])
+def test_keypair_counter_and_seed_forms_agree(mode):
+    from cudilithium_amd.api import Dilithium, keypair_device
+    n = 200
+    pk, sk = Dilithium(mode).keypair(n, rand_index=1234)
+    pk2, sk2 = keypair_device(mode, n=n, rand_index=1234)
+    epk, esk = oracle.keypair_counter_batch(mode, n, rand_index=1234)
+    assert np.array_equal(pk, epk) and np.array_equal(sk, esk)
+    assert np.array_equal(pk2, epk) and np.array_equal(sk2, esk)
+
+
+@pytest.mark.parametrize("mode", [2, 3, 5])
+def test_verify_bit_flips_match_oracle(mode):
+    from cudilithium_amd.api import Dilithium
+    n = 256
+    rng = np.random.default_rng(100 + mode)
+    pk, sk = oracle.keypair_counter_batch(mode, n, rand_index=5)
+    m = rng.integers(0, 256, (n, 40), dtype=np.uint8)
+    sig, _ = oracle.sign_batch(mode, m, sk)
+    flip = rng.random(n) < 0.5
+    for i in np.nonzero(flip)[0]:
+        bit = rng.integers(0, sig.shape[1] * 8)
+        sig[i, bit // 8] ^= 1 << (bit % 8)
+    ret = Dilithium(mode).verify(sig, m, pk)
+    exp = oracle.verify_batch(mode, sig, m, pk)
+    assert np.array_equal(ret, exp)
+    assert (ret[~flip] == 0).all()
+    assert (ret[flip] == -1).mean() > 0.95
+
+
+def test_verify_wrong_siglen_rejects_all():
+    from cudilithium_amd.api import Dilithium
+    d = Dilithium(3)
+    pk, sk = oracle.keypair_counter_batch(3, 8)
+    m = np.zeros((8, 32), np.uint8)
+    sig, _ = oracle.sign_batch(3, m, sk)
+    assert (d.verify(sig, m, pk, siglen=d.sig_bytes - 1) == -1).all()
+
+
+@pytest.mark.parametrize("mode", [2, 5])
+def test_verify_malformed_hints(mode):
+    """Non-increasing hint indices, a count above omega, a decreasing count
+    and non-zero padding are all rejected, as the oracle does."""
+    from cudilithium_amd.api import Dilithium
+    p = oracle.params(mode)
+    n = 16
+    pk, sk = oracle.keypair_counter_batch(mode, n, rand_index=9)
+    m = np.arange(n * 16, dtype=np.uint8).reshape(n, 16)
+    sig, _ = oracle.sign_batch(mode, m, sk)
+    hoff = 32 + p.l * p.polyz_bytes
+    bad = sig.copy()
+    for i in range(n):
+        h = bad[i, hoff:]
+        kind = i % 4
+        cnt = int(h[p.omega + p.k - 1])
+        if kind == 0 and cnt >= 2:
+            h[1] = h[0]  # not strictly increasing (first polynomial)
+        elif kind == 1:
+            h[p.omega] = p.omega + 1  # count beyond omega
+        elif kind == 2 and cnt < p.omega:
+            h[p.omega - 1] = 7  # padding not zero
+        else:
+            h[p.omega + 1] = 0 if h[p.omega] > 0 else 1  # decreasing / inconsistent counts
+    ret = Dilithium(mode).verify(bad, m, pk)
+    exp = oracle.verify_batch(mode, bad, m, pk)
+    assert np.array_equal(ret, exp)
+
+
+def test_verify_device_variable_message_lengths():
+    from cudilithium_amd.api import verify_device
+    mode = 5
+    n = 64
+    rng = np.random.default_rng(3)
+    pk, sk = oracle.keypair_counter_batch(mode, n, rand_index=11)
+    msgs = [rng.integers(0, 256, int(rng.integers(0, 3000)), dtype=np.uint8).tobytes() for _ in range(n)]
+    msgs[0] = b""
+    sig, _ = oracle.sign_batch(mode, msgs, sk)
+    assert (verify_device(mode, sig, msgs, pk) == 0).all()
+    msgs2 = list(msgs)
+    for i in range(0, n, 3):
+        msgs2[i] = msgs2[i] + b"x"
+    ret = verify_device(mode, sig, msgs2, pk)
+    exp = oracle.verify_batch(mode, sig, msgs2, pk)
+    assert np.array_equal(ret, exp)
