@@ -1,0 +1,261 @@
+"""Benchmark of the batched Dilithium engine on one MI355X.
+
+Headline workload (default): Dilithium3 deterministic signing of 16384
+(key, message) pairs, message lengths log-uniform in [32 B, 4 KiB], keys from
+seeds, inputs resident in HBM, one fixed-stage pass plus one persistent
+rejection-loop launch per step.  A step signs the whole batch.
+
+Printed on one JSON line: throughput (signatures/s), the roofline of the
+dominant kernel (the persistent rejection loop, integer-VALU bound), the
+oracle's CPU rate on a bounded sample, and the per-stage (reference
+architecture) GPU form on the same inputs.
+
+    python bench.py [--steps K] [--warmup W] [--workload sign3|sign2|sign5|verify5|ntt]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "signs/s & verifies/s per MI355X (Dil2/3/5), bit-exact; serving p50/p99 latency"
+PEAK_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # CUs x SIMDs x lanes x GHz = 39.3 T int32 ops/s
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def bench_sign(args, mode: int):
+    from cudilithium_amd import api, workmodel, workloads
+    from cudilithium_amd.device import DeviceBuffer, synchronize
+
+    n = args.batch or 16384
+    seeds, msgs = workloads.config3(n) if mode == 3 else (
+        workloads.xoshiro_bytes(workloads.CONFIG_SEED[3] + mode, n * 32).reshape(n, 32),
+        workloads.messages_loguniform(workloads.CONFIG_SEED[3] + 1000 + mode, n, 32, 4096))
+    pkb, skb, sigb = api.sizes(mode)
+    log(f"[bench] setup: mode {mode}, {n} keys from seeds, messages {sum(map(len, msgs))} bytes")
+    pk, sk = api.keypair_device(mode, seeds=seeds)
+    d_sk = DeviceBuffer.from_array(sk)
+    d_msgs = api.DeviceMessages(msgs)
+    d_sig = DeviceBuffer(n * sigb)
+    nslots = args.slots or n
+    ctx = api.SignContext(mode, n, nslots)
+    ctx.set_tiling(args.spb, 0)
+
+    for _ in range(args.warmup):
+        ctx.sign_device(d_sig, d_sk, d_msgs, api.ENGINE_PERSISTENT)
+    synchronize()
+    ctx.enable_timing(args.steps)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.sign_device(d_sig, d_sk, d_msgs, api.ENGINE_PERSISTENT)
+    synchronize()
+    wall = time.perf_counter() - t0
+    tim = ctx.timings()
+    ctx.enable_timing(0)
+    sig = d_sig.download((n, sigb), np.uint8)
+    attempts = ctx.attempt_counts(n)
+    log(f"[bench] persistent: {args.steps} steps in {wall * 1e3:.2f} ms; loop kernel "
+        f"{np.mean([t[1] for t in tim]):.3f} ms, fixed stage {np.mean([t[0] for t in tim]):.3f} ms")
+
+    # parity on a sample (outside the timed region)
+    import oracle
+    rng = np.random.default_rng(1)
+    idx = np.sort(rng.choice(n, size=min(n, args.check), replace=False))
+    exp, exp_att = oracle.sign_batch(mode, [msgs[i] for i in idx], sk[idx])
+    parity = bool(np.array_equal(sig[idx], exp)) and bool(np.array_equal(attempts[idx], exp_att))
+    log(f"[bench] parity vs oracle on {len(idx)} items: {parity}")
+
+    # roofline of the rejection loop: algorithmic int32 ops of the attempts the
+    # specification makes, over the loop kernel's event-timed duration
+    loop_ms = float(np.mean([t[1] for t in tim]))
+    ops = float(attempts.sum()) * workmodel.sign_attempt_ops(mode)
+    achieved = ops / (loop_ms * 1e-3) / 1e12
+    value = n * args.steps / wall
+
+    perstage = None
+    if not args.no_perstage:
+        thr = 2048
+        ps = api.SignContext(mode, n, thr)
+        ps.sign_device(d_sig, d_sk, d_msgs, api.ENGINE_PERSTAGE)
+        synchronize()
+        reps = max(1, min(args.steps, 3))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ps.sign_device(d_sig, d_sk, d_msgs, api.ENGINE_PERSTAGE)
+        synchronize()
+        ps_wall = (time.perf_counter() - t0) / reps
+        ps_sig = d_sig.download((n, sigb), np.uint8)
+        perstage = {"value": n / ps_wall, "unit": "signs/s", "exec_threshold": thr,
+                    "ms_per_step": ps_wall * 1e3, "bit_exact_vs_persistent": bool(np.array_equal(ps_sig, sig))}
+        log(f"[bench] per-stage form: {ps_wall * 1e3:.2f} ms per batch")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_sign(mode, msgs, sk, args.cpu_seconds)
+
+    cfg = {"workload": f"dilithium{mode}_sign_{n}_msg_loguniform_32B-4KiB",
+           "batch": n, "mode": mode, "engine": "persistent", "slots": nslots,
+           "slots_per_block": args.spb or "auto", "mean_attempts": float(attempts.mean())}
+    return {
+        "metric": METRIC, "value": value, "unit": "signs/s", "ms_per_step": wall / args.steps * 1e3,
+        "dtype": "int32", "config": cfg, "parity": "bit-exact" if parity else "MISMATCH",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
+                     "frac": achieved / PEAK_INT32_TOPS, "traffic": None,
+                     "kernel": "sg_persistent_kernel", "kernel_ms": loop_ms,
+                     "algorithmic_ops": ops},
+        "cpu_baseline": cpu, "perstage_gpu_baseline": perstage,
+        "speedup_vs_perstage": (value / perstage["value"]) if perstage else None,
+    }
+
+
+def cpu_baseline_sign(mode, msgs, sk, seconds):
+    """The oracle (OpenMP over items, all host cores) on the first items of
+    the same workload, sized to about `seconds` of wall time."""
+    import oracle
+    threads = oracle.max_threads()
+    t0 = time.perf_counter()
+    oracle.sign_batch(mode, msgs[:64], sk[:64])
+    per = (time.perf_counter() - t0) / 64
+    m = int(max(64, min(len(msgs), seconds / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    oracle.sign_batch(mode, msgs[:m], sk[:m])
+    dt = time.perf_counter() - t0
+    return {"value": m / dt, "unit": "signs/s", "cores": threads, "kind": "port",
+            "sample": f"first {m} of the batch's (key, message) pairs, OpenMP schedule(dynamic)"}
+
+
+def bench_verify(args, mode: int = 5):
+    from cudilithium_amd import api, workmodel, workloads
+    from cudilithium_amd.device import DeviceBuffer, synchronize
+    import ctypes
+
+    n = args.batch or 65536
+    seeds, msgs, flip_idx, flip_bit = workloads.config4(n)
+    pkb, skb, sigb = api.sizes(mode)
+    mlist = [m.tobytes() for m in msgs]
+    pk, sk = api.keypair_device(mode, seeds=seeds)
+    ctx = api.SignContext(mode, n, n)
+    sig = ctx.sign(mlist, sk)
+    for i, b in zip(flip_idx, flip_bit):
+        sig[i, b // 8] ^= 1 << (b % 8)
+    d_sig, d_pk = DeviceBuffer.from_array(sig), DeviceBuffer.from_array(pk)
+    dm = api.DeviceMessages(mlist)
+    wrow = api.L().dil_verify_work_row_bytes(mode)
+    d_work, d_ret = DeviceBuffer(n * wrow), DeviceBuffer(4 * n)
+    L = api.L()
+
+    def run():
+        api._lib.check(L.dil_verify_device(mode, d_ret.ptr, d_sig.ptr, sigb, d_pk.ptr, pkb, dm.data.ptr, dm.offs.ptr,
+                                           dm.lens.ptr, n, d_work.ptr, wrow, None), "verify")
+    for _ in range(args.warmup):
+        run()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    synchronize()
+    wall = time.perf_counter() - t0
+    ret = d_ret.download((n,), np.int32)
+    exp = np.zeros(n, np.int32)
+    exp[flip_idx] = -1
+    import oracle
+    sample = np.random.default_rng(2).choice(n, 256, replace=False)
+    oret = oracle.verify_batch(mode, sig[sample], [mlist[i] for i in sample], pk[sample])
+    parity = bool((ret[sample] == oret).all()) and bool((ret[np.setdiff1d(np.arange(n), flip_idx)] == 0).all())
+    value = n * args.steps / wall
+    ops = n * workmodel.verify_ops(mode, 1024)
+    achieved = ops / (wall / args.steps) / 1e12
+    return {"metric": METRIC, "value": value, "unit": "verifies/s", "ms_per_step": wall / args.steps * 1e3,
+            "dtype": "int32", "parity": "bit-exact" if parity else "MISMATCH",
+            "config": {"workload": f"dilithium{mode}_verify_{n}_msg_1KiB_10pct_bitflips", "batch": n, "mode": mode,
+                       "rejected": int((ret != 0).sum())},
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
+                         "frac": achieved / PEAK_INT32_TOPS, "traffic": None, "kernel": "verify pipeline (3 launches)"},
+            "cpu_baseline": None}
+
+
+def bench_ntt(args):
+    from cudilithium_amd import ntt, workloads, workmodel
+    from cudilithium_amd.device import DeviceBuffer, Event, synchronize
+    n = args.batch or 65536
+    a, b = workloads.config2(n)
+    da, db, dc = DeviceBuffer.from_array(a), DeviceBuffer.from_array(b), DeviceBuffer(a.nbytes)
+    for _ in range(args.warmup):
+        ntt.pipeline_device(dc, da, db, n)
+    synchronize()
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(args.steps):
+        ntt.pipeline_device(dc, da, db, n)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    import oracle
+    c = dc.download(a.shape, np.int32)
+    idx = np.random.default_rng(3).choice(n, 256, replace=False)
+    parity = bool(np.array_equal(c[idx], oracle.ntt_pipeline(a[idx], b[idx])))
+    ops = n * workmodel.NTT_PIPELINE_OPS
+    return {"metric": METRIC, "value": n / (ms * 1e-3), "unit": "polys/s", "ms_per_step": ms, "dtype": "int32",
+            "parity": "bit-exact" if parity else "MISMATCH",
+            "config": {"workload": f"ntt_pipeline_{n}x256", "batch": n},
+            "roofline": {"bound": "valu", "achieved": ops / (ms * 1e-3) / 1e12, "peak": PEAK_INT32_TOPS,
+                         "unit": "TOP/s", "frac": ops / (ms * 1e-3) / 1e12 / PEAK_INT32_TOPS,
+                         "traffic": None, "hbm_gbs": n * 3072 / (ms * 1e-3) / 1e9},
+            "cpu_baseline": None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="sign3")
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--slots", type=int, default=0, help="attempt slots (default: batch)")
+    ap.add_argument("--spb", type=int, default=0, help="slots per workgroup (0 = automatic)")
+    ap.add_argument("--check", type=int, default=128, help="items checked against the oracle")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-perstage", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(shard batches across devices by item range instead)", file=sys.stderr)
+        sys.exit(2)
+    from cudilithium_amd.device import device_name
+    log(f"[bench] device: {device_name()}; host threads {cpu_threads()}")
+    w = args.workload
+    if w.startswith("sign"):
+        res = bench_sign(args, int(w[4:] or 3))
+    elif w.startswith("verify"):
+        res = bench_verify(args, int(w[6:] or 5))
+    elif w == "ntt":
+        res = bench_ntt(args)
+    else:
+        raise SystemExit(f"unknown workload {w}")
+    res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "data": "synthetic"})
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -55,6 +55,17 @@ def _register():
     R("dil_keypair_work_row_bytes", c_size_t, [c_int])
     R("dil_keypair_device", c_int, [c_int, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t,
                                     ctypes.c_uint64, c_size_t, c_void_p, c_size_t, c_void_p])
+    R("dil_sign_ctx_create", c_int, [ctypes.POINTER(c_void_p), c_int, c_size_t, c_size_t])
+    R("dil_sign_ctx_destroy", c_int, [c_void_p])
+    R("dil_sign_ctx_set_tiling", c_int, [c_void_p, c_int, c_int, c_int])
+    R("dil_sign_ctx_enable_timing", c_int, [c_void_p, c_int])
+    R("dil_sign_ctx_timing", c_int, [c_void_p, c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)])
+    R("dil_sign_ctx_timed_calls", c_int, [c_void_p])
+    R("dil_sign_device", c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p,
+                                 c_size_t, c_int, c_void_p])
+    R("dil_sign_ctx_attempt_counts", c_int, [c_void_p, c_void_p, c_size_t, c_void_p])
+    R("dil_sign_ctx_status", c_int, [c_void_p, ctypes.POINTER(c_int), c_void_p])
+    R("dil_sign_ctx_executed_attempts", c_int, [c_void_p, ctypes.POINTER(ctypes.c_uint64), c_void_p])
     R("dil_verify_work_row_bytes", c_size_t, [c_int])
     R("dil_verify_device", c_int, [c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p,
                                    c_void_p, c_size_t, c_void_p, c_size_t, c_void_p])
@@ -221,3 +232,81 @@ def verify_device(mode: int, sig: np.ndarray, msgs, pk: np.ndarray):
     _lib.check(L().dil_verify_device(mode, d_ret.ptr, d_sig.ptr, sig.shape[1], d_pk.ptr, pk.shape[1], d_msg.ptr,
                                      d_off.ptr, d_len.ptr, n, d_work.ptr, wrow, None), "verify_device")
     return d_ret.download((n,), np.int32)
+
+
+class DeviceMessages:
+    """A batch of messages resident in device memory (flat bytes plus per-item
+    8-byte-aligned offsets and lengths)."""
+
+    def __init__(self, msgs):
+        flat, offs, lens = _pack_msgs(msgs)
+        self.n = len(msgs)
+        self.total_bytes = int(lens.sum())
+        self.data, self.offs, self.lens = (DeviceBuffer.from_array(flat), DeviceBuffer.from_array(offs),
+                                           DeviceBuffer.from_array(lens))
+
+
+class SignContext:
+    """dil_sign_ctx: per-item rows and attempt slots for batches of up to
+    max_batch signatures of one mode."""
+
+    def __init__(self, mode: int, max_batch: int, nslots: int):
+        self.mode, self.max_batch, self.nslots = mode, max_batch, nslots
+        self.pk_bytes, self.sk_bytes, self.sig_bytes = sizes(mode)
+        p = c_void_p()
+        _lib.check(L().dil_sign_ctx_create(ctypes.byref(p), mode, max_batch, nslots), "sign_ctx_create")
+        self.handle = p.value
+
+    def set_tiling(self, slots_per_block: int = 0, occupancy: int = 0, blocks: int = 0) -> None:
+        _lib.check(L().dil_sign_ctx_set_tiling(self.handle, slots_per_block, occupancy, blocks), "set_tiling")
+
+    def enable_timing(self, max_calls: int) -> None:
+        _lib.check(L().dil_sign_ctx_enable_timing(self.handle, int(max_calls)), "enable_timing")
+
+    def timings(self):
+        """[(fixed_ms, loop_ms)] of the timed calls since enable_timing."""
+        out = []
+        for i in range(L().dil_sign_ctx_timed_calls(self.handle)):
+            f, l = ctypes.c_float(), ctypes.c_float()
+            _lib.check(L().dil_sign_ctx_timing(self.handle, i, ctypes.byref(f), ctypes.byref(l)), "timing")
+            out.append((f.value, l.value))
+        return out
+
+    def attempt_counts(self, n: int) -> np.ndarray:
+        out = np.zeros(max(1, n), np.uint32)
+        _lib.check(L().dil_sign_ctx_attempt_counts(self.handle, out.ctypes.data, n, None), "attempt_counts")
+        return out[:n]
+
+    def sign_device(self, d_sig: DeviceBuffer, d_sk: DeviceBuffer, msgs: DeviceMessages,
+                    engine: int = ENGINE_PERSISTENT, stream=None) -> None:
+        _lib.check(L().dil_sign_device(self.handle, d_sig.ptr, self.sig_bytes, d_sk.ptr, self.sk_bytes,
+                                       msgs.data.ptr, msgs.offs.ptr, msgs.lens.ptr, msgs.n, engine, stream),
+                   "sign_device")
+
+    def executed_attempts(self) -> int:
+        v = ctypes.c_uint64(0)
+        _lib.check(L().dil_sign_ctx_executed_attempts(self.handle, ctypes.byref(v), None), "executed_attempts")
+        return v.value
+
+    def status(self) -> int:
+        s = c_int(0)
+        _lib.check(L().dil_sign_ctx_status(self.handle, ctypes.byref(s), None), "status")
+        return s.value
+
+    def sign(self, msgs, sk: np.ndarray, engine: int = ENGINE_PERSISTENT) -> np.ndarray:
+        n = len(msgs)
+        dm = DeviceMessages(msgs)
+        d_sk = DeviceBuffer.from_array(np.ascontiguousarray(sk, dtype=np.uint8))
+        d_sig = DeviceBuffer(n * self.sig_bytes)
+        self.sign_device(d_sig, d_sk, dm, engine)
+        if engine == ENGINE_PERSISTENT and self.status() != 0:
+            raise _lib.DilError("persistent engine stopped at its round limit")
+        return d_sig.download((n, self.sig_bytes), np.uint8)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                L().dil_sign_ctx_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

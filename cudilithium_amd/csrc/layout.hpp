@@ -46,8 +46,9 @@ struct SignLayout {
     using SL = SlotLayout<P>;
     static constexpr size_t SIG = 0;
     static constexpr size_t SK = align256(Z::SIG_BYTES);
-    static constexpr size_t MU = SK + align256(Z::SK_BYTES);  // mu (64) | rhoprime (64)
+    static constexpr size_t MU = SK + align256(Z::SK_BYTES);  // mu (64) | rhoprime (64) | attempts (4)
     static constexpr size_t RHOPRIME = MU + CRHBYTES;
+    static constexpr size_t ATTEMPTS = MU + 2 * CRHBYTES;
     static constexpr size_t SLOT = MU + 256;
     static constexpr size_t M = SLOT + SL::BYTES;
     static size_t ints(size_t mlen) { return M + align256(mlen); }

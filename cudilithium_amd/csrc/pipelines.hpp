@@ -68,5 +68,10 @@ int launch_sign_copy(int mode, const SignArgs &a, const SlotRef &sr, const uint6
 int sign_perstage(int mode, const SignArgs &a, const SlotRef &item_slots, const SlotRef &temp_slots, size_t thr,
                   SignLutEntry *sign_lut, uint32_t *h_exec, uint32_t *d_exec, uint8_t *h_done, uint8_t *d_done,
                   uint64_t *h_copy, uint64_t *d_copy, hipStream_t st);
+// Rejection loop of the persistent engine over nslots slots (the fixed part
+// must have been launched first).  blocks <= 0: nslots / slots_per_block.
+int launch_sign_persistent(int mode, const SignArgs &a, const SlotRef &sr, size_t nslots, int slots_per_block,
+                           int occupancy, int blocks, uint32_t *d_head, hipStream_t st);
+int default_slots_per_block(size_t nslots);
 
 }  // namespace dil

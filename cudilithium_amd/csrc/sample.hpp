@@ -86,18 +86,22 @@ __device__ __forceinline__ void eta_poly(const uint64_t (&seed)[8], uint32_t non
     int ctr = 0;
     do {
         keccak_f1600(s);
-        static_for<0, 2 * SHAKE256_RATE>([&](auto c) {
-            constexpr int C = decltype(c)::value;
-            const uint32_t t = block_bits<4 * C, 4>(s);
-            if constexpr (ETA == 2) {
-                if (t < 15u && ctr < N) {
-                    sink(ctr, 2 - (int32_t)(t - ((205u * t) >> 10) * 5u));
-                    ++ctr;
-                }
-            } else {
-                if (t < 9u && ctr < N) {
-                    sink(ctr, 4 - (int32_t)t);
-                    ++ctr;
+        // 17 state words, 16 nibbles each, low nibble first
+        static_for<0, 2 * (SHAKE256_RATE / 8)>([&](auto h) {
+            uint32_t w = half_word<decltype(h)::value>(s);
+#pragma unroll 2
+            for (int b = 0; b < 8; ++b, w >>= 4) {
+                const uint32_t t = w & 15u;
+                if constexpr (ETA == 2) {
+                    if (t < 15u && ctr < N) {
+                        sink(ctr, 2 - (int32_t)(t - ((205u * t) >> 10) * 5u));
+                        ++ctr;
+                    }
+                } else {
+                    if (t < 9u && ctr < N) {
+                        sink(ctr, 4 - (int32_t)t);
+                        ++ctr;
+                    }
                 }
             }
         });
@@ -135,10 +139,17 @@ __device__ __forceinline__ void challenge(const uint64_t (&ct)[4], int8_t *cp) {
             ++i;
         }
     };
-    static_for<8, SHAKE256_RATE>([&](auto p) { step(block_bits<8 * decltype(p)::value, 8>(s)); });
+    // bytes 8.. of the first block (the first 8 are the sign bits), then
+    // further blocks if ever needed; 4 bytes per 32-bit half-word
+    auto bytes_of = [&](auto h) {
+        uint32_t w = half_word<decltype(h)::value>(s);
+#pragma unroll 1
+        for (int b = 0; b < 4; ++b, w >>= 8) step(w & 0xFFu);
+    };
+    static_for<2, SHAKE256_RATE / 4>(bytes_of);
     while (i < N) {
         keccak_f1600(s);
-        static_for<0, SHAKE256_RATE>([&](auto p) { step(block_bits<8 * decltype(p)::value, 8>(s)); });
+        static_for<0, SHAKE256_RATE / 4>(bytes_of);
     }
 }
 

@@ -1,0 +1,34 @@
+// Persistent signing engine: mode dispatch and workgroup tiling defaults.
+#include "sign_persistent.hpp"
+
+namespace dil {
+
+// control words: [0] queue head, [1] status, [2] attempts executed
+__global__ void zero_ctl_kernel(uint32_t *p) {
+    p[0] = 0;
+    p[1] = 0;
+    p[2] = 0;
+}
+
+int launch_sign_persistent(int mode, const SignArgs &a, const SlotRef &sr, size_t nslots, int slots_per_block,
+                           int occupancy, int blocks, uint32_t *d_head, hipStream_t st) {
+    if (a.n == 0) return 0;
+    if (occupancy == 0) occupancy = 2;
+    switch (mode) {
+        case 2: return launch_persistent_p<Params<2>>(a, sr, nslots, slots_per_block, occupancy, blocks, d_head, st);
+        case 3: return launch_persistent_p<Params<3>>(a, sr, nslots, slots_per_block, occupancy, blocks, d_head, st);
+        case 5: return launch_persistent_p<Params<5>>(a, sr, nslots, slots_per_block, occupancy, blocks, d_head, st);
+        default: return -1;
+    }
+}
+
+// Slots per workgroup used when the caller does not choose: the largest
+// supported value that still leaves one workgroup per compute unit.
+int default_slots_per_block(size_t nslots) {
+    const size_t cus = (size_t)device_cu_count();
+    for (int s : {32, 16})
+        if (nslots / s >= cus) return s;
+    return nslots >= 8 ? 8 : 1;
+}
+
+}  // namespace dil

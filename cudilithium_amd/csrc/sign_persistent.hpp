@@ -1,0 +1,190 @@
+// Persistent work-queue signing engine.
+//
+// Replaces the reference's host rejection scheduler (api.cu:208-441: one
+// stream synchronisation and ~8 host<->device transfers per round) with one
+// launch in which every workgroup loops until the batch is done:
+//
+//   refill   claim items from a global queue (one atomicAdd per refill) until
+//            the workgroup owns S undone items or the queue is empty
+//   assign   S attempt slots round-robin over the owned items: item o gets
+//            slots o, o + U, o + 2U, ... with consecutive nonces, so freed
+//            slots turn into speculative nonces of unfinished items instead of
+//            idling at the end of the batch
+//   y w c z  the four attempt stages on the S slots (sign_stages.hpp), with
+//            the lane mapping each stage wants, separated by barriers
+//   commit   per item, the first successful slot in slot order is the
+//            smallest successful nonce -- the sequential specification's
+//            result -- and its signature is copied out
+//
+// Workgroups never wait on each other (the queue is only read-modify-written
+// by atomicAdd), so there is no co-residency requirement and no deadlock: a
+// workgroup that starts after the queue has drained exits at once.  Every
+// round gives each owned item at least one attempt, so each workgroup
+// finishes with probability 1, as the reference's loop does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+#include "sign_stages.hpp"
+
+namespace dil {
+
+constexpr int PERSIST_THREADS = 256;
+constexpr uint32_t PERSIST_MAX_ROUNDS = 20000;
+
+// WPE: waves per SIMD the register allocation must allow (512 / WPE VGPRs)
+template <class P, int S, int WPE>
+__global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void sg_persistent_kernel(
+    SignArgs a, SlotRef sr, uint32_t *head) {
+    using Z = Sizes<P>;
+    __shared__ ZetaLds zt;
+    __shared__ int32_t x[PERSIST_THREADS / 64][NTT_XBUF];
+    __shared__ uint32_t own_item[S], own_next[S];
+    __shared__ int own_done[S];
+    __shared__ uint32_t slot_own[S], slot_nonce[S];
+    __shared__ int slot_ok[S];
+    __shared__ int n_own, queue_empty;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+    constexpr int NW = PERSIST_THREADS / 64;
+    const size_t slot0 = (size_t)blockIdx.x * S;
+    auto slot_base = [&](int s) { return sr.base + (slot0 + s) * sr.pitch; };
+    auto slot_sig = [&](int s) { return sr.sig + (slot0 + s) * sr.sig_pitch; };
+    zeta_lds_load(zt);
+    if (tid == 0) {
+        n_own = 0;
+        queue_empty = 0;
+    }
+    __syncthreads();
+
+    for (uint32_t round = 0;; ++round) {
+        if (round == PERSIST_MAX_ROUNDS) {
+            // no correct run gets here (each round succeeds per item with
+            // probability >= 1/5); stop instead of spinning and report it
+            if (tid == 0) atomicOr(head + 1, 1u);
+            break;
+        }
+        if (tid == 0) {
+            int u = n_own;
+            if (u < S && !queue_empty) {
+                const uint32_t start = atomicAdd(head, (uint32_t)(S - u));
+                for (int j = 0; j < S - u && (size_t)start + j < a.n; ++j) {
+                    own_item[n_own] = start + j;
+                    own_next[n_own] = 0;
+                    ++n_own;
+                }
+                if ((size_t)start + (S - u) >= a.n) queue_empty = 1;
+            }
+            u = n_own;
+            for (int s = 0; s < S && u > 0; ++s) {
+                const int o = s % u;
+                slot_own[s] = o;
+                slot_nonce[s] = own_next[o] + s / u;
+            }
+            for (int o = 0; o < u; ++o) {
+                own_next[o] += (S - o + u - 1) / u;
+                own_done[o] = 0;
+            }
+        }
+        __syncthreads();
+        const int u = n_own;
+        if (u == 0) break;
+        if (tid == 0) atomicAdd(head + 2, (uint32_t)S);  // attempts executed (incl. speculative)
+
+        for (int t = tid; t < S * P::L; t += PERSIST_THREADS) {
+            const int s = t / P::L;
+            stage_y<P>(a, own_item[slot_own[s]], slot_nonce[s], t % P::L, slot_base(s));
+        }
+        __syncthreads();
+        for (int s = wave; s < S; s += NW) stage_w<P>(a, own_item[slot_own[s]], slot_base(s), x[wave], zt);
+        __syncthreads();
+        for (int s = tid; s < S; s += PERSIST_THREADS) stage_c<P>(a, own_item[slot_own[s]], slot_base(s), slot_sig(s));
+        __syncthreads();
+        for (int s = wave; s < S; s += NW) {
+            const bool ok = stage_z<P>(a, own_item[slot_own[s]], slot_base(s), slot_sig(s), x[wave], zt);
+            if (lane == 0) slot_ok[s] = ok ? 1 : 0;
+        }
+        __syncthreads();
+        // commit: slots of item o are o, o + u, ... in increasing nonce order
+        for (int o = wave; o < u; o += NW) {
+            int win = -1;
+            for (int s = o; s < S; s += u)
+                if (slot_ok[s]) {
+                    win = s;
+                    break;
+                }
+            if (win >= 0) {
+                const uint8_t *src = slot_sig(win);
+                uint8_t *dst = a.sig + (size_t)own_item[o] * a.sig_pitch;
+                for (int b = lane; b < Z::SIG_BYTES; b += 64) dst[b] = src[b];
+                if (lane == 0) {
+                    own_done[o] = 1;
+                    // attempts the sequential specification makes for this item
+                    *reinterpret_cast<uint32_t *>(a.rows + (size_t)own_item[o] * a.row_pitch +
+                                                  SignLayout<P>::ATTEMPTS) = slot_nonce[win] + 1;
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int k = 0;
+            for (int o = 0; o < u; ++o)
+                if (!own_done[o]) {
+                    own_item[k] = own_item[o];
+                    own_next[k] = own_next[o];
+                    ++k;
+                }
+            n_own = k;
+        }
+        __syncthreads();
+    }
+}
+
+// queue head and status word; a kernel rather than a memset so that the
+// sequence can be captured in a HIP graph
+__global__ void zero_ctl_kernel(uint32_t *p);
+
+template <class P, int S, int WPE>
+static int launch_persistent_s(const SignArgs &a, const SlotRef &sr, size_t nslots, int blocks, uint32_t *d_head,
+                               hipStream_t st) {
+    int nb = (int)(nslots / S);
+    if (blocks > 0 && blocks < nb) nb = blocks;
+    if (nb < 1) return -3;
+    hipLaunchKernelGGL(zero_ctl_kernel, dim3(1), dim3(1), 0, st, d_head);
+    hipLaunchKernelGGL((sg_persistent_kernel<P, S, WPE>), dim3(nb), dim3(PERSIST_THREADS), 0, st, a, sr, d_head);
+    return (int)hipGetLastError();
+}
+
+// Defined once per mode (sign_persistent_<mode>.hip).  occupancy: waves per
+// SIMD the kernel's register allocation allows (2, 3 or 4).
+template <class P>
+int launch_persistent_p(const SignArgs &a, const SlotRef &sr, size_t nslots, int slots_per_block, int occupancy,
+                        int blocks, uint32_t *d_head, hipStream_t st);
+
+template <class P, int WPE>
+static int launch_persistent_w(const SignArgs &a, const SlotRef &sr, size_t nslots, int slots_per_block, int blocks,
+                               uint32_t *d_head, hipStream_t st) {
+    switch (slots_per_block) {
+        case 1: return launch_persistent_s<P, 1, WPE>(a, sr, nslots, blocks, d_head, st);
+        case 8: return launch_persistent_s<P, 8, WPE>(a, sr, nslots, blocks, d_head, st);
+        case 16: return launch_persistent_s<P, 16, WPE>(a, sr, nslots, blocks, d_head, st);
+        case 32: return launch_persistent_s<P, 32, WPE>(a, sr, nslots, blocks, d_head, st);
+        default: return -1;
+    }
+}
+
+#define DIL_DEFINE_PERSISTENT_LAUNCHER(MODE)                                                                       \
+    template <>                                                                                                  \
+    int launch_persistent_p<Params<MODE>>(const SignArgs &a, const SlotRef &sr, size_t nslots,                    \
+                                          int slots_per_block, int occupancy, int blocks, uint32_t *d_head,       \
+                                          hipStream_t st) {                                                      \
+        using P = Params<MODE>;                                                                                  \
+        switch (occupancy) {                                                                                     \
+            case 2: return launch_persistent_w<P, 2>(a, sr, nslots, slots_per_block, blocks, d_head, st);        \
+            case 3: return launch_persistent_w<P, 3>(a, sr, nslots, slots_per_block, blocks, d_head, st);        \
+            case 4: return launch_persistent_w<P, 4>(a, sr, nslots, slots_per_block, blocks, d_head, st);        \
+            default: return -1;                                                                                  \
+        }                                                                                                        \
+    }
+
+}  // namespace dil

@@ -187,6 +187,41 @@ int dil_verify_device(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig
                       const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, uint8_t *d_work,
                       size_t work_pitch, void *stream);
 
+/* Signing context: owns the per-item rows (dil_sign_pool_row_bytes(mode, 0)
+ * each), nslots attempt slots and the scheduler tables of either engine. */
+typedef struct dil_sign_ctx dil_sign_ctx;
+int dil_sign_ctx_create(dil_sign_ctx **ctx, int mode, size_t max_batch, size_t nslots);
+int dil_sign_ctx_destroy(dil_sign_ctx *ctx);
+/* Persistent engine tiling: slots per workgroup (0 = automatic, or 1, 8, 16,
+ * 32), register budget as waves per SIMD (0 = automatic, or 2, 3, 4) and
+ * workgroup count (0 = nslots / slots_per_block). */
+int dil_sign_ctx_set_tiling(dil_sign_ctx *ctx, int slots_per_block, int occupancy, int blocks);
+/* Records HIP events on the call's stream around the fixed stage and the
+ * rejection loop of the next max_calls dil_sign_device calls (0 turns timing
+ * off and forgets earlier records).  dil_sign_ctx_timing waits for call
+ * `call` (0-based) and returns its two durations. */
+int dil_sign_ctx_enable_timing(dil_sign_ctx *ctx, int max_calls);
+int dil_sign_ctx_timed_calls(dil_sign_ctx *ctx);
+int dil_sign_ctx_timing(dil_sign_ctx *ctx, int call, float *fixed_ms, float *loop_ms);
+/* Deterministic signatures of n messages (per-item lengths) under n secret
+ * keys, all in device memory; engine DIL_ENGINE_PERSISTENT or
+ * DIL_ENGINE_PERSTAGE (n <= 65536).  Asynchronous for the persistent engine;
+ * the per-stage engine synchronises `stream` once per rejection round. */
+int dil_sign_device(dil_sign_ctx *ctx, uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_sk,
+                    size_t sk_pitch, const uint8_t *d_msg, const uint64_t *d_moff,
+                    const uint32_t *d_mlen, size_t n, int engine, void *stream);
+/* After a persistent-engine call: per item, the number of attempts the
+ * sequential specification makes (winning nonce + 1), copied to host and
+ * synchronised on `stream`. */
+int dil_sign_ctx_attempt_counts(dil_sign_ctx *ctx, uint32_t *h_counts, size_t n, void *stream);
+/* Status of the last persistent-engine call (synchronises `stream`): 0, or
+ * non-zero if a workgroup stopped at its round limit (never expected; the
+ * batch's signatures are then incomplete). */
+int dil_sign_ctx_status(dil_sign_ctx *ctx, int *status, void *stream);
+/* Attempts the last persistent-engine call executed, speculative ones
+ * included (synchronises `stream`). */
+int dil_sign_ctx_executed_attempts(dil_sign_ctx *ctx, uint64_t *count, void *stream);
+
 /* ---------------------------------------------------------------------- */
 /* Standalone NTT (include/ntt.cuh:30-62, src/ntt.cu:77-207)              */
 /* Polynomials are 256 int32 coefficients; consecutive polynomials start  */

@@ -144,3 +144,50 @@ def test_verify_device_variable_message_lengths():
     ret = verify_device(mode, sig, msgs2, pk)
     exp = oracle.verify_batch(mode, sig, msgs2, pk)
     assert np.array_equal(ret, exp)
+
+
+@pytest.mark.parametrize("mode", [2, 3, 5])
+def test_persistent_engine_matches_oracle_and_perstage(mode):
+    """Device-resident batch with per-item message lengths (0 B .. 5 KiB):
+    the persistent engine, the per-stage engine and the oracle agree byte for
+    byte, for several slot tilings (including ones that force speculative
+    nonces and many queue refills)."""
+    from cudilithium_amd.api import ENGINE_PERSISTENT, ENGINE_PERSTAGE, SignContext
+    rng = np.random.default_rng(40 + mode)
+    n = 333
+    pk, sk = oracle.keypair_counter_batch(mode, n, rand_index=321)
+    msgs = [rng.integers(0, 256, int(rng.integers(0, 5000)), dtype=np.uint8).tobytes() for _ in range(n)]
+    msgs[1] = b""
+    exp, att = oracle.sign_batch(mode, msgs, sk)
+    for nslots, spb, occ in [(64, 8, 2), (256, 16, 3), (1024, 32, 4), (7, 1, 2), (2048, 32, 2)]:
+        ctx = SignContext(mode, n, nslots)
+        ctx.set_tiling(spb, occ)
+        got = ctx.sign(msgs, sk, ENGINE_PERSISTENT)
+        assert np.array_equal(got, exp), (nslots, spb, occ)
+    ctx = SignContext(mode, n, 100)
+    assert np.array_equal(ctx.sign(msgs, sk, ENGINE_PERSTAGE), exp)
+    assert (oracle.verify_batch(mode, exp, msgs, pk) == 0).all()
+
+
+def test_persistent_engine_repeated_calls_and_small_batches():
+    """The context is reusable: the queue head and slot state are reset per
+    call; batches of 1, 2 and 65 items."""
+    from cudilithium_amd.api import SignContext
+    mode = 3
+    pk, sk = oracle.keypair_counter_batch(mode, 65, rand_index=2)
+    ctx = SignContext(mode, 65, 512)
+    for n in (1, 2, 65, 2):
+        msgs = [bytes([n, i]) * 20 for i in range(n)]
+        exp, _ = oracle.sign_batch(mode, msgs, sk[:n])
+        assert np.array_equal(ctx.sign(msgs, sk[:n]), exp)
+
+
+def test_dropin_persistent_matches_perstage_full_rounds():
+    from cudilithium_amd.api import ENGINE_PERSISTENT, ENGINE_PERSTAGE, Dilithium
+    d = Dilithium(2)
+    pk, sk = oracle.keypair_counter_batch(2, 500, rand_index=8)
+    m = np.random.default_rng(8).integers(0, 256, (500, 64), dtype=np.uint8)
+    a = d.sign(m, sk, exec_threshold=128, engine=ENGINE_PERSISTENT)
+    b = d.sign(m, sk, exec_threshold=128, engine=ENGINE_PERSTAGE)
+    assert np.array_equal(a, b)
+    assert np.array_equal(a, oracle.sign_batch(2, m, sk)[0])
