@@ -57,7 +57,27 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_cxx_tests()
     return LIB
+
+
+def build_cxx_tests() -> None:
+    """The C++ drop-in KAT program (tests/cxx/kat_dropin.cpp), one binary per
+    mode, linked against the in-tree library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cxx", "kat_dropin.cpp")
+    out_dir = os.path.join(root, "tests", "cxx", "bin")
+    if not os.path.exists(src):
+        return
+    os.makedirs(out_dir, exist_ok=True)
+    for mode in (2, 3, 5):
+        out = os.path.join(out_dir, f"kat_dropin_{mode}")
+        cmd = [HIPCC, "-std=c++17", "-O2", f"-DDILITHIUM_MODE={mode}", "-Wno-unused-value", "-Wno-unused-result",
+               f"-I{os.path.join(root, 'include')}", src, f"-L{HERE}", "-lcudilithium_amd",
+               "-Wl,-rpath,$ORIGIN/../../../cudilithium_amd", "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {out} failed:\n{r.stderr}")
 
 
 if __name__ == "__main__":

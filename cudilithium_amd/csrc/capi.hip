@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/cudilithium_amd.h"
 #include "internal.hpp"
@@ -21,6 +22,25 @@ int device_cu_count() {
         cached[dev] = cus;
     }
     return cached[dev];
+}
+
+AuxStreams *aux_streams() {
+    static AuxStreams *per_dev[64] = {nullptr};
+    static std::mutex mu;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    if (!per_dev[dev]) {
+        AuxStreams *a = new AuxStreams();
+        bool ok = hipStreamCreateWithFlags(&a->s[0], hipStreamNonBlocking) == hipSuccess &&
+                  hipStreamCreateWithFlags(&a->s[1], hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&a->fork, hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&a->join[0], hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&a->join[1], hipEventDisableTiming) == hipSuccess;
+        if (!ok) return nullptr;
+        per_dev[dev] = a;
+    }
+    return per_dev[dev];
 }
 
 int default_grid_blocks(size_t units, int per_block, int blocks_per_cu) {
@@ -82,6 +102,11 @@ int dil_ntt_pointwise_pipeline(int32_t *d_c, const int32_t *d_a, const int32_t *
                                size_t pitch_words, void *stream) {
     if ((npoly && (!d_c || !d_a || !d_b)) || pitch_words < (size_t)N || (pitch_words & 3)) return DIL_EINVAL;
     return launch_ntt(2, d_c, d_a, d_b, npoly, pitch_words, 0, S(stream));
+}
+
+int dil_valu_microbench(int op, uint32_t iters, double *lane_ops_per_s) {
+    if (op < 0 || op > 7 || !lane_ops_per_s || iters == 0) return DIL_EINVAL;
+    return valu_microbench(op, iters, lane_ops_per_s);
 }
 
 int dil_keccak_f1600_batch(uint64_t *d_states, size_t n, void *stream) {

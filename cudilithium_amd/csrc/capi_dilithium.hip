@@ -310,17 +310,17 @@ extern "C" {
 
 int dil_sign_ctx_destroy(dil_sign_ctx *c) {
     if (!c) return 0;
-    hipFree(c->rows);
-    hipFree(c->slots);
-    hipFree(c->d_head);
-    hipFree(c->d_exec);
-    hipFree(c->d_done);
-    hipFree(c->d_copy);
-    hipHostFree(c->h_exec);
-    hipHostFree(c->h_done);
-    hipHostFree(c->h_copy);
+    (void)hipFree(c->rows);
+    (void)hipFree(c->slots);
+    (void)hipFree(c->d_head);
+    (void)hipFree(c->d_exec);
+    (void)hipFree(c->d_done);
+    (void)hipFree(c->d_copy);
+    (void)hipHostFree(c->h_exec);
+    (void)hipHostFree(c->h_done);
+    (void)hipHostFree(c->h_copy);
     for (auto &e : c->ev)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     delete c;
     return 0;
 }

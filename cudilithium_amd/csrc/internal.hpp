@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 
 #include "keccak.hpp"
 #include "params.hpp"
@@ -19,9 +20,21 @@ int device_cu_count();
 // blocks per CU.
 int default_grid_blocks(size_t units, int per_block, int blocks_per_cu = 8);
 
+// Two auxiliary streams (default priority, non-blocking) and fork/join
+// events of the current device, created on first use and kept until the
+// process ends.  A caller holds `lock` while it enqueues a fork/join so that
+// host threads sharing the device cannot interleave their event records.
+struct AuxStreams {
+    hipStream_t s[2];
+    hipEvent_t fork, join[2];
+    std::mutex lock;
+};
+AuxStreams *aux_streams();
+
 // Launchers (return a hipError_t value, 0 on success).
 int launch_ntt(int kind, int32_t *c, const int32_t *a, const int32_t *b, size_t npoly, size_t pitch_words,
                int blocks, hipStream_t stream);
 int launch_keccak(uint64_t *st, size_t n, hipStream_t stream);
+int valu_microbench(int op, uint32_t iters, double *ops_per_s);
 
 }  // namespace dil

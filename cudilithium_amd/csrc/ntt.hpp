@@ -62,10 +62,10 @@ constexpr ZetaConsts kZeta = make_zetas();
 static_assert(kZeta.z[1] == 25847 && kZeta.z[2] == -2608894 && kZeta.z[255] == 1976782,
               "zetas closed form");
 
-// LDS copy of the zetas used by every NTT in a workgroup
+// LDS copy of the zetas used by every NTT in a workgroup: {z, zq} pairs so
+// that one 64-bit LDS read fetches both
 struct ZetaLds {
-    int32_t z[N];
-    int32_t zq[N];
+    int2 zz[N];
 };
 
 __device__ inline const ZetaConsts &zeta_consts() {
@@ -75,10 +75,7 @@ __device__ inline const ZetaConsts &zeta_consts() {
 
 __device__ __forceinline__ void zeta_lds_load(ZetaLds &t) {
     const ZetaConsts &c = zeta_consts();
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        t.z[i] = c.z[i];
-        t.zq[i] = c.zq[i];
-    }
+    for (int i = threadIdx.x; i < N; i += blockDim.x) t.zz[i] = make_int2(c.z[i], c.zq[i]);
 }
 
 // Orders LDS traffic between lanes of one wavefront.
@@ -96,15 +93,36 @@ constexpr int NTT_XBUF = N + N / 16;
 __device__ __forceinline__ int xpad(int j) { return j + (j >> 4); }
 
 __device__ __forceinline__ void ct_bfly(int32_t &a, int32_t &b, const ZetaLds &zt, int k) {
-    const int32_t t = mont_mul_tw(b, zt.z[k], zt.zq[k]);
+    const int2 z = zt.zz[k];
+    const int32_t t = mont_mul_tw(b, z.x, z.y);
     b = a - t;
     a = a + t;
 }
 
 __device__ __forceinline__ void gs_bfly(int32_t &a, int32_t &b, const ZetaLds &zt, int k) {
+    const int2 z = zt.zz[k];
     const int32_t t = a;
     a = t + b;
-    b = mont_mul_tw(t - b, -zt.z[k], (int32_t)(0u - (uint32_t)zt.zq[k]));
+    b = mont_mul_tw(t - b, -z.x, (int32_t)(0u - (uint32_t)z.y));
+}
+
+// A <-> B without LDS: swaps register bit 1 with lane bit 5 (upper half of
+// one register with the lower half of its partner) and register bit 0 with
+// lane bit 4 (odd 16-lane rows with even rows), gfx950's v_permlane32_swap
+// and v_permlane16_swap.
+__device__ __forceinline__ void swap_AB(int32_t (&v)[4]) {
+    auto p0 = __builtin_amdgcn_permlane32_swap(v[0], v[2], false, false);
+    auto p1 = __builtin_amdgcn_permlane32_swap(v[1], v[3], false, false);
+    v[0] = p0[0];
+    v[2] = p0[1];
+    v[1] = p1[0];
+    v[3] = p1[1];
+    auto q0 = __builtin_amdgcn_permlane16_swap(v[0], v[1], false, false);
+    auto q1 = __builtin_amdgcn_permlane16_swap(v[2], v[3], false, false);
+    v[0] = q0[0];
+    v[1] = q0[1];
+    v[2] = q1[0];
+    v[3] = q1[1];
 }
 
 __device__ __forceinline__ int idx_A(int l, int r) { return 64 * r + l; }
@@ -130,7 +148,7 @@ __device__ __forceinline__ void ntt_wave(int32_t (&v)[4], int32_t *x, const Zeta
     ct_bfly(v[1], v[3], zt, 1);
     ct_bfly(v[0], v[1], zt, 2);
     ct_bfly(v[2], v[3], zt, 3);
-    transpose(v, x, l, idx_A, idx_B);
+    swap_AB(v);
     {
         const int h = l >> 4;  // j >> 6
         ct_bfly(v[0], v[2], zt, 4 + h);
@@ -177,7 +195,7 @@ __device__ __forceinline__ void intt_wave(int32_t (&v)[4], int32_t *x, const Zet
         gs_bfly(v[0], v[2], zt, 7 - h);
         gs_bfly(v[1], v[3], zt, 7 - h);
     }
-    transpose(v, x, l, idx_B, idx_A);
+    swap_AB(v);
     gs_bfly(v[0], v[1], zt, 3);
     gs_bfly(v[2], v[3], zt, 2);
     gs_bfly(v[0], v[2], zt, 1);

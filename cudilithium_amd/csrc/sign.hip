@@ -113,15 +113,32 @@ __global__ __launch_bounds__(256) void sg_copy_kernel(SignArgs a, SlotRef sr, co
     }
 }
 
+// The three fixed-stage launches are independent: mu/rhoprime (one lane per
+// item, latency-bound on long messages), ExpandA (one lane per matrix entry,
+// throughput-bound) and the s1/s2/t0 NTTs run as a fork/join over the
+// caller's stream and two auxiliary streams of this device, so the idle
+// lanes of one fill with the others' waves.  Event fork/join is also what a
+// HIP graph capture records as parallel branches.
 template <class P>
 int sign_fixed_impl(const SignArgs &a, hipStream_t st) {
     if (a.n == 0) return 0;
     constexpr int KL = P::K * P::L, NP = P::L + 2 * P::K;
-    hipLaunchKernelGGL(sg_mu_kernel<P>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(sg_expand_kernel<P>, dim3((unsigned)((a.n * KL + 255) / 256)), dim3(256), 0, st, a);
+    AuxStreams *aux = aux_streams();
+    if (!aux) return (int)hipErrorOutOfMemory;
+    std::lock_guard<std::mutex> guard(aux->lock);
+    int e;
+    if ((e = (int)hipEventRecord(aux->fork, st))) return e;
+    if ((e = (int)hipStreamWaitEvent(aux->s[0], aux->fork, 0))) return e;
+    if ((e = (int)hipStreamWaitEvent(aux->s[1], aux->fork, 0))) return e;
+    hipLaunchKernelGGL(sg_mu_kernel<P>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, aux->s[0], a);
     hipLaunchKernelGGL(sg_prep_kernel<P>, dim3(default_grid_blocks(a.n * NP, SIGN_WAVES)), dim3(64 * SIGN_WAVES), 0,
-                       st, a);
-    return (int)hipGetLastError();
+                       aux->s[1], a);
+    hipLaunchKernelGGL(sg_expand_kernel<P>, dim3((unsigned)((a.n * KL + 255) / 256)), dim3(256), 0, st, a);
+    if ((e = (int)hipGetLastError())) return e;
+    if ((e = (int)hipEventRecord(aux->join[0], aux->s[0]))) return e;
+    if ((e = (int)hipEventRecord(aux->join[1], aux->s[1]))) return e;
+    if ((e = (int)hipStreamWaitEvent(st, aux->join[0], 0))) return e;
+    return (int)hipStreamWaitEvent(st, aux->join[1], 0);
 }
 
 template <class P>

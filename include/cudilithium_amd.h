@@ -243,6 +243,12 @@ int dil_ntt_pointwise_pipeline(int32_t *d_c, const int32_t *d_a, const int32_t *
  * d_states[j * n + i]).  Device pointer. */
 int dil_keccak_f1600_batch(uint64_t *d_states, size_t n, void *stream);
 
+/* Measured lane-operations per second of one VALU instruction class on the
+ * current device (synchronous): 0 add_u32, 1 bitop3_b32, 2 mul_lo_u32,
+ * 3 mul_hi_i32, 4 fma_f64, 5 alignbit_b32, 6 one (cvt_f64_i32, mul_f64,
+ * cvt_i32_f64, add) sequence, 7 mul_u32_u24. */
+int dil_valu_microbench(int op, uint32_t iters, double *lane_ops_per_s);
+
 /* ---------------------------------------------------------------------- */
 /* Runtime helpers                                                        */
 /* ---------------------------------------------------------------------- */
