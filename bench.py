@@ -55,9 +55,10 @@ def bench_sign(args, mode: int):
     d_sk = DeviceBuffer.from_array(sk)
     d_msgs = api.DeviceMessages(msgs)
     d_sig = DeviceBuffer(n * sigb)
-    nslots = args.slots or n
-    ctx = api.SignContext(mode, n, nslots)
-    ctx.set_tiling(args.spb, 0)
+    ctx = api.SignContext(mode, n, args.slots or None)  # autotuned tiling unless overridden
+    nslots = ctx.nslots
+    if args.spb or args.occupancy:
+        ctx.set_tiling(args.spb, args.occupancy)
 
     for _ in range(args.warmup):
         ctx.sign_device(d_sig, d_sk, d_msgs, api.ENGINE_PERSISTENT)
@@ -114,7 +115,8 @@ def bench_sign(args, mode: int):
 
     cfg = {"workload": f"dilithium{mode}_sign_{n}_msg_loguniform_32B-4KiB",
            "batch": n, "mode": mode, "engine": "persistent", "slots": nslots,
-           "slots_per_block": args.spb or "auto", "mean_attempts": float(attempts.mean())}
+           "slots_per_block": ctx.slots_per_block or "auto", "occupancy": ctx.occupancy or "auto",
+           "mean_attempts": float(attempts.mean())}
     return {
         "metric": METRIC, "value": value, "unit": "signs/s", "ms_per_step": wall / args.steps * 1e3,
         "dtype": "int32", "config": cfg, "parity": "bit-exact" if parity else "MISMATCH",
@@ -231,7 +233,8 @@ def main():
     ap.add_argument("--workload", default="sign3")
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--slots", type=int, default=0, help="attempt slots (default: batch)")
-    ap.add_argument("--spb", type=int, default=0, help="slots per workgroup (0 = automatic)")
+    ap.add_argument("--spb", type=int, default=0, help="slots per workgroup (0 = autotuned)")
+    ap.add_argument("--occupancy", type=int, default=0, help="waves per SIMD of the loop kernel (0 = autotuned)")
     ap.add_argument("--check", type=int, default=128, help="items checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")

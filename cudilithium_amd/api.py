@@ -250,15 +250,27 @@ class SignContext:
     """dil_sign_ctx: per-item rows and attempt slots for batches of up to
     max_batch signatures of one mode."""
 
-    def __init__(self, mode: int, max_batch: int, nslots: int):
+    def __init__(self, mode: int, max_batch: int, nslots: int | None = None):
+        """nslots None: the autotuner's choice for this mode and batch bucket
+        (cudilithium_amd/tuning.json), else one slot per item."""
+        from . import autotune
+        tuned = autotune.lookup(mode, max_batch)
+        if nslots is None:
+            nslots = max(1, int(max_batch * (tuned[0] if tuned else 1.0)))
         self.mode, self.max_batch, self.nslots = mode, max_batch, nslots
         self.pk_bytes, self.sk_bytes, self.sig_bytes = sizes(mode)
         p = c_void_p()
         _lib.check(L().dil_sign_ctx_create(ctypes.byref(p), mode, max_batch, nslots), "sign_ctx_create")
         self.handle = p.value
+        if tuned and nslots // tuned[1] >= 1:
+            self.set_tiling(tuned[1], tuned[2])
+
+    slots_per_block = 0
+    occupancy = 0
 
     def set_tiling(self, slots_per_block: int = 0, occupancy: int = 0, blocks: int = 0) -> None:
         _lib.check(L().dil_sign_ctx_set_tiling(self.handle, slots_per_block, occupancy, blocks), "set_tiling")
+        self.slots_per_block, self.occupancy = slots_per_block, occupancy
 
     def enable_timing(self, max_calls: int) -> None:
         _lib.check(L().dil_sign_ctx_enable_timing(self.handle, int(max_calls)), "enable_timing")

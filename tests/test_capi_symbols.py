@@ -13,6 +13,21 @@ def test_library_exports_all_declared_symbols():
     assert not missing, missing
 
 
+def test_pool_rows_fit_reference_formulas():
+    """Pools sized with the reference callers' formulas
+    (tests/test_vectors.cu:64-70, 81-100, 117-128, 149-166) are large enough."""
+    from cudilithium_amd import api
+    L = api.L()
+    for mode in (2, 3, 5):
+        assert L.dil_keypair_pool_row_bytes(mode) <= api.reference_row_bytes(mode, api.POOL_KEYPAIR)
+        assert L.dil_sign_temp_row_bytes(mode) <= api.reference_row_bytes(mode, api.POOL_SIGN_TEMP)
+        for mlen in (0, 1, 32, 33, 64, 1000, 4096, 65536):
+            assert L.dil_sign_pool_row_bytes(mode, mlen) <= api.reference_row_bytes(mode, api.POOL_SIGN, mlen)
+            assert L.dil_verify_pool_row_bytes(mode, mlen) <= api.reference_row_bytes(mode, api.POOL_VERIFY, mlen)
+    # the reference formula itself, Dilithium3 with 32-byte messages
+    assert api.reference_row_bytes(3, api.POOL_SIGN, 32) == 80896
+
+
 def test_sizes_match_reference_params():
     L = _lib.load()
     # include/params.h:71-73
