@@ -10,7 +10,7 @@ dominant kernel (the persistent rejection loop, integer-VALU bound), the
 oracle's CPU rate on a bounded sample, and the per-stage (reference
 architecture) GPU form on the same inputs.
 
-    python bench.py [--steps K] [--warmup W] [--workload sign3|sign2|sign5|verify5|ntt]
+    python bench.py [--steps K] [--warmup W] [--workload sign3|sign2|sign5|verify5|ntt|copy]
 """
 from __future__ import annotations
 
@@ -28,6 +28,13 @@ sys.path.insert(0, HERE)
 METRIC = "signs/s & verifies/s per MI355X (Dil2/3/5), bit-exact; serving p50/p99 latency"
 PEAK_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # CUs x SIMDs x lanes x GHz = 39.3 T int32 ops/s
 PEAK_HBM_GBS = 8000.0
+
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by a 1 GiB copy's readings (read = 2 x FETCH_SIZE,
+# written = WRITE_SIZE); profiles/sign3_16384_hbm_valu_counters.md.
+TRAFFIC_BYTES = {
+    "dilithium3_sign_16384_msg_loguniform_32B-4KiB": (2 * 2241688 + 1317875) * 1024,
+}
 
 
 def log(*a):
@@ -121,7 +128,7 @@ def bench_sign(args, mode: int):
         "metric": METRIC, "value": value, "unit": "signs/s", "ms_per_step": wall / args.steps * 1e3,
         "dtype": "int32", "config": cfg, "parity": "bit-exact" if parity else "MISMATCH",
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
-                     "frac": achieved / PEAK_INT32_TOPS, "traffic": None,
+                     "frac": achieved / PEAK_INT32_TOPS, "traffic": TRAFFIC_BYTES.get(cfg["workload"]),
                      "kernel": "sg_persistent_kernel", "kernel_ms": loop_ms,
                      "algorithmic_ops": ops},
         "cpu_baseline": cpu, "perstage_gpu_baseline": perstage,
@@ -225,6 +232,34 @@ def bench_ntt(args):
             "cpu_baseline": None}
 
 
+def bench_copy(args):
+    """Plain device-to-device copy of `batch` MiB (default 1024): the calibration
+    kernel for the FETCH_SIZE / WRITE_SIZE counters and a check of HBM bandwidth."""
+    from cudilithium_amd import api
+    from cudilithium_amd.device import DeviceBuffer, Event, synchronize
+    nbytes = (args.batch or 1024) << 20
+    src, dst = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+    L = api.L()
+    api._lib.check(L.dil_memset_device(src.ptr, 0x5A, nbytes, None), "memset")
+    for _ in range(args.warmup):
+        api._lib.check(L.dil_memcpy_d2d(dst.ptr, src.ptr, nbytes, None), "copy")
+    synchronize()
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(args.steps):
+        api._lib.check(L.dil_memcpy_d2d(dst.ptr, src.ptr, nbytes, None), "copy")
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    return {"metric": "device copy bandwidth (calibration)", "value": gbs, "unit": "GB/s", "ms_per_step": ms,
+            "dtype": "u8", "config": {"workload": f"d2d_copy_{nbytes >> 20}MiB", "bytes_read": nbytes,
+                                      "bytes_written": nbytes},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": gbs / PEAK_HBM_GBS, "traffic": None},
+            "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +288,8 @@ def main():
         res = bench_verify(args, int(w[6:] or 5))
     elif w == "ntt":
         res = bench_ntt(args)
+    elif w == "copy":
+        res = bench_copy(args)
     else:
         raise SystemExit(f"unknown workload {w}")
     res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,

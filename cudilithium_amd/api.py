@@ -69,6 +69,29 @@ def _register():
     R("dil_verify_work_row_bytes", c_size_t, [c_int])
     R("dil_verify_device", c_int, [c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p,
                                    c_void_p, c_size_t, c_void_p, c_size_t, c_void_p])
+    pp = ctypes.POINTER(c_void_p)
+    R("dil_sign_graph_create", c_int, [pp, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p,
+                                       c_void_p, c_size_t])
+    R("dil_verify_graph_create", c_int, [pp, c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p,
+                                         c_void_p, c_void_p, c_size_t, c_void_p, c_size_t])
+    R("dil_keypair_graph_create", c_int, [pp, c_int, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t,
+                                          ctypes.c_uint64, c_size_t, c_void_p, c_size_t])
+    R("dil_keystore_row_bytes", c_size_t, [c_int])
+    R("dil_keystore_create", c_int, [pp, c_int, c_size_t])
+    R("dil_keystore_destroy", c_int, [c_void_p])
+    R("dil_keystore_capacity", c_size_t, [c_void_p])
+    R("dil_keystore_mode", c_int, [c_void_p])
+    R("dil_keystore_load", c_int, [c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t, c_void_p])
+    R("dil_keystore_keygen", c_int, [c_void_p, c_size_t, c_void_p, c_size_t, ctypes.c_uint64, c_size_t, c_void_p])
+    R("dil_keystore_export", c_int, [c_void_p, c_size_t, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p])
+    R("dil_sign_ctx_create_resident", c_int, [pp, c_int, c_size_t, c_size_t])
+    R("dil_sign_device_resident", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p,
+                                          c_void_p, c_size_t, c_int, c_void_p])
+    R("dil_verify_device_resident", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p,
+                                            c_void_p, c_size_t, c_void_p, c_size_t, c_void_p])
+    R("dil_graph_launch", c_int, [c_void_p, c_void_p])
+    R("dil_graph_batch", c_size_t, [c_void_p])
+    R("dil_graph_destroy", c_int, [c_void_p])
     _registered = True
 
 
@@ -250,9 +273,10 @@ class SignContext:
     """dil_sign_ctx: per-item rows and attempt slots for batches of up to
     max_batch signatures of one mode."""
 
-    def __init__(self, mode: int, max_batch: int, nslots: int | None = None):
+    def __init__(self, mode: int, max_batch: int, nslots: int | None = None, resident_only: bool = False):
         """nslots None: the autotuner's choice for this mode and batch bucket
-        (cudilithium_amd/tuning.json), else one slot per item."""
+        (cudilithium_amd/tuning.json), else one slot per item.  resident_only:
+        rows without per-item expanded arrays (sign_resident only)."""
         from . import autotune
         tuned = autotune.lookup(mode, max_batch)
         if nslots is None:
@@ -260,7 +284,8 @@ class SignContext:
         self.mode, self.max_batch, self.nslots = mode, max_batch, nslots
         self.pk_bytes, self.sk_bytes, self.sig_bytes = sizes(mode)
         p = c_void_p()
-        _lib.check(L().dil_sign_ctx_create(ctypes.byref(p), mode, max_batch, nslots), "sign_ctx_create")
+        create = L().dil_sign_ctx_create_resident if resident_only else L().dil_sign_ctx_create
+        _lib.check(create(ctypes.byref(p), mode, max_batch, nslots), "sign_ctx_create")
         self.handle = p.value
         if tuned and nslots // tuned[1] >= 1:
             self.set_tiling(tuned[1], tuned[2])
@@ -295,6 +320,31 @@ class SignContext:
                                        msgs.data.ptr, msgs.offs.ptr, msgs.lens.ptr, msgs.n, engine, stream),
                    "sign_device")
 
+    def sign_resident_device(self, store: "KeyStore", d_key_idx: DeviceBuffer, d_sig: DeviceBuffer,
+                             msgs: DeviceMessages, engine: int = ENGINE_PERSISTENT, stream=None) -> None:
+        _lib.check(L().dil_sign_device_resident(self.handle, store.handle, d_key_idx.ptr, d_sig.ptr, self.sig_bytes,
+                                                msgs.data.ptr, msgs.offs.ptr, msgs.lens.ptr, msgs.n, engine, stream),
+                   "sign_device_resident")
+
+    def sign_resident(self, store: "KeyStore", key_idx, msgs, engine: int = ENGINE_PERSISTENT) -> np.ndarray:
+        n = len(msgs)
+        dm = DeviceMessages(msgs)
+        d_idx = DeviceBuffer.from_array(np.ascontiguousarray(key_idx, dtype=np.uint32))
+        d_sig = DeviceBuffer(n * self.sig_bytes)
+        self.sign_resident_device(store, d_idx, d_sig, dm, engine)
+        if engine == ENGINE_PERSISTENT and self.status() != 0:
+            raise _lib.DilError("persistent engine stopped at its round limit")
+        return d_sig.download((n, self.sig_bytes), np.uint8)
+
+    def graph(self, d_sig: DeviceBuffer, d_sk: DeviceBuffer, msgs: DeviceMessages) -> "Graph":
+        """The persistent-engine call sign_device(d_sig, d_sk, msgs) captured
+        as a HIP graph (replayed by Graph.launch)."""
+        p = c_void_p()
+        _lib.check(L().dil_sign_graph_create(ctypes.byref(p), self.handle, d_sig.ptr, self.sig_bytes, d_sk.ptr,
+                                             self.sk_bytes, msgs.data.ptr, msgs.offs.ptr, msgs.lens.ptr, msgs.n),
+                   "sign_graph_create")
+        return Graph(p.value, keep=(self, d_sig, d_sk, msgs))
+
     def executed_attempts(self) -> int:
         v = ctypes.c_uint64(0)
         _lib.check(L().dil_sign_ctx_executed_attempts(self.handle, ctypes.byref(v), None), "executed_attempts")
@@ -322,3 +372,241 @@ class SignContext:
                 self.handle = None
         except Exception:
             pass
+
+
+class Graph:
+    """dil_graph: one captured keygen / sign / verify call.  Holds references
+    to the buffers it was captured with so they outlive it."""
+
+    def __init__(self, handle: int, keep=()):
+        self.handle, self._keep = handle, keep
+        self.n = L().dil_graph_batch(handle)
+
+    def launch(self, stream=None) -> None:
+        _lib.check(L().dil_graph_launch(self.handle, stream), "graph_launch")
+
+    def __del__(self):
+        try:
+            if self.handle:
+                L().dil_graph_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def verify_graph(mode: int, d_ret: DeviceBuffer, d_sig: DeviceBuffer, d_pk: DeviceBuffer, msgs: DeviceMessages,
+                 d_work: DeviceBuffer, work_pitch: int) -> Graph:
+    """dil_verify_device over these buffers captured as a HIP graph."""
+    pkb, _, sigb = sizes(mode)
+    p = c_void_p()
+    _lib.check(L().dil_verify_graph_create(ctypes.byref(p), mode, d_ret.ptr, d_sig.ptr, sigb, d_pk.ptr, pkb,
+                                           msgs.data.ptr, msgs.offs.ptr, msgs.lens.ptr, msgs.n, d_work.ptr,
+                                           work_pitch), "verify_graph_create")
+    return Graph(p.value, keep=(d_ret, d_sig, d_pk, msgs, d_work))
+
+
+def keypair_graph(mode: int, d_pk: DeviceBuffer, d_sk: DeviceBuffer, n: int, d_work: DeviceBuffer, work_pitch: int,
+                  d_seeds: DeviceBuffer | None = None, rand_index: int = 0) -> Graph:
+    """dil_keypair_device over these buffers captured as a HIP graph."""
+    pkb, skb, _ = sizes(mode)
+    p = c_void_p()
+    _lib.check(L().dil_keypair_graph_create(ctypes.byref(p), mode, d_pk.ptr, pkb, d_sk.ptr, skb,
+                                            d_seeds.ptr if d_seeds else None, 32, rand_index, n, d_work.ptr,
+                                            work_pitch), "keypair_graph_create")
+    return Graph(p.value, keep=(d_pk, d_sk, d_work, d_seeds))
+
+
+class KeyStore:
+    """dil_keystore: `capacity` resident keys of one mode with their expanded
+    A, s1^, s2^, t0^, t1^ and tr."""
+
+    def __init__(self, mode: int, capacity: int):
+        p = c_void_p()
+        _lib.check(L().dil_keystore_create(ctypes.byref(p), mode, capacity), "keystore_create")
+        self.handle, self.mode, self.capacity = p.value, mode, capacity
+        self.pk_bytes, self.sk_bytes, self.sig_bytes = sizes(mode)
+
+    def load(self, first: int, pk: np.ndarray, sk: np.ndarray | None = None) -> None:
+        """Install host keys at [first, first + len(pk)) and expand them."""
+        pk = np.ascontiguousarray(pk, dtype=np.uint8)
+        skp = np.ascontiguousarray(sk, dtype=np.uint8) if sk is not None else None
+        _lib.check(L().dil_keystore_load(self.handle, first, pk.ctypes.data, self.pk_bytes,
+                                         skp.ctypes.data if skp is not None else None, self.sk_bytes, pk.shape[0],
+                                         None), "keystore_load")
+        stream_sync(None)
+
+    def keygen(self, first: int, seeds: np.ndarray | None = None, n: int | None = None, rand_index: int = 0) -> None:
+        d_seeds = None
+        if seeds is not None:
+            seeds = np.ascontiguousarray(seeds, dtype=np.uint8).reshape(-1, 32)
+            n = seeds.shape[0]
+            d_seeds = DeviceBuffer.from_array(seeds)
+        _lib.check(L().dil_keystore_keygen(self.handle, first, d_seeds.ptr if d_seeds else None, 32, rand_index, n,
+                                           None), "keystore_keygen")
+        stream_sync(None)
+
+    def export(self, first: int, n: int):
+        pk = np.zeros((n, self.pk_bytes), np.uint8)
+        sk = np.zeros((n, self.sk_bytes), np.uint8)
+        _lib.check(L().dil_keystore_export(self.handle, first, n, pk.ctypes.data, self.pk_bytes, sk.ctypes.data,
+                                           self.sk_bytes, None), "keystore_export")
+        stream_sync(None)
+        return pk, sk
+
+    def verify(self, key_idx, sig: np.ndarray, msgs) -> np.ndarray:
+        n = len(msgs)
+        dm = DeviceMessages(msgs)
+        d_idx = DeviceBuffer.from_array(np.ascontiguousarray(key_idx, dtype=np.uint32))
+        d_sig = DeviceBuffer.from_array(np.ascontiguousarray(sig, dtype=np.uint8))
+        wrow = L().dil_verify_work_row_bytes(self.mode)
+        d_work, d_ret = DeviceBuffer(n * wrow), DeviceBuffer(4 * n)
+        _lib.check(L().dil_verify_device_resident(self.handle, d_idx.ptr, d_ret.ptr, d_sig.ptr, sig.shape[1],
+                                                  dm.data.ptr, dm.offs.ptr, dm.lens.ptr, n, d_work.ptr, wrow, None),
+                   "verify_device_resident")
+        return d_ret.download((n,), np.int32)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                L().dil_keystore_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ serving
+
+OP_KEYGEN, OP_SIGN, OP_VERIFY = 0, 1, 2
+
+
+class ServerConfig(ctypes.Structure):  # dil_server_config
+    _fields_ = [("hbm_budget_bytes", c_size_t), ("lanes", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("max_keys", ctypes.c_uint32 * 3), ("batch_window_us", ctypes.c_uint32)]
+
+
+class Request(ctypes.Structure):  # dil_request
+    _fields_ = [("op", ctypes.c_int32), ("mode", ctypes.c_int32), ("key", ctypes.c_uint32), ("mlen", ctypes.c_uint32),
+                ("msg", c_void_p), ("seed", c_void_p), ("sig", c_void_p), ("out", c_void_p),
+                ("status", ctypes.c_int32), ("verdict", ctypes.c_int32),
+                ("t_submit_ns", ctypes.c_uint64), ("t_launch_ns", ctypes.c_uint64), ("t_done_ns", ctypes.c_uint64),
+                ("batch_id", ctypes.c_uint32), ("done", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):  # dil_server_stats
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("items_by_op", ctypes.c_uint64 * 3), ("lane_bytes", ctypes.c_uint64),
+                ("max_msg_bytes", ctypes.c_uint64)]
+
+
+def _register_server():
+    R = _lib.register
+    pp = ctypes.POINTER(c_void_p)
+    R("dil_server_create", c_int, [pp, ctypes.POINTER(ServerConfig)])
+    R("dil_server_destroy", c_int, [c_void_p])
+    R("dil_server_keystore", c_void_p, [c_void_p, c_int])
+    R("dil_server_submit", c_int, [c_void_p, c_void_p, c_size_t])
+    R("dil_server_wait", c_int, [c_void_p, c_void_p, ctypes.c_int64])
+    R("dil_server_drain", c_int, [c_void_p])
+    R("dil_server_get_stats", c_int, [c_void_p, ctypes.POINTER(ServerStats)])
+    R("dil_server_replay", c_int, [c_void_p, c_void_p, c_void_p, c_size_t])
+    R("dil_monotonic_ns", ctypes.c_uint64, [])
+
+
+class _StoreView(KeyStore):
+    """A server's key store (owned by the server, not freed here)."""
+
+    def __init__(self, handle, mode, capacity):  # noqa: super().__init__ allocates; this wraps
+        self.handle, self.mode, self.capacity = handle, mode, capacity
+        self.pk_bytes, self.sk_bytes, self.sig_bytes = sizes(mode)
+
+    def __del__(self):
+        pass
+
+
+class Server:
+    """dil_server: continuous batching of keygen / sign / verify requests with
+    resident keys.  Requests are `Request` structs whose buffers the caller
+    keeps alive until done (Server.requests builds such arrays)."""
+
+    def __init__(self, hbm_budget_bytes: int, lanes: int = 3, max_batch: int = 4096, max_keys=(0, 0, 0),
+                 batch_window_us: int = 0):
+        L()
+        _register_server()
+        cfg = ServerConfig(hbm_budget_bytes, lanes, max_batch, (ctypes.c_uint32 * 3)(*max_keys), batch_window_us)
+        p = c_void_p()
+        _lib.check(L().dil_server_create(ctypes.byref(p), ctypes.byref(cfg)), "server_create")
+        self.handle = p.value
+        self.stores = {}
+        for i, m in enumerate((2, 3, 5)):
+            if max_keys[i]:
+                self.stores[m] = _StoreView(L().dil_server_keystore(self.handle, m), m, max_keys[i])
+
+    def submit(self, reqs) -> None:
+        """reqs: a ctypes array of Request (all submitted at once)."""
+        ptrs = (c_void_p * len(reqs))(*[ctypes.addressof(r) for r in reqs])
+        _lib.check(L().dil_server_submit(self.handle, ptrs, len(reqs)), "server_submit")
+
+    def replay(self, reqs, arrival_ns: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arrival_ns, dtype=np.uint64)
+        _lib.check(L().dil_server_replay(self.handle, ctypes.addressof(reqs), arr.ctypes.data, len(reqs)),
+                   "server_replay")
+
+    def drain(self) -> None:
+        _lib.check(L().dil_server_drain(self.handle), "server_drain")
+
+    def wait(self, req, timeout_us: int = -1) -> int:
+        return L().dil_server_wait(self.handle, ctypes.addressof(req), timeout_us)
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _lib.check(L().dil_server_get_stats(self.handle, ctypes.byref(st)), "server_stats")
+        return st
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            L().dil_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RequestBatch:
+    """A ctypes array of n Requests plus the host buffers they point to:
+    messages (concatenated), signatures to verify, seeds, and outputs."""
+
+    def __init__(self, ops, modes, keys, msgs, sigs=None, seeds=None):
+        n = len(ops)
+        self.n = n
+        self.reqs = (Request * n)()
+        self._msgs = [np.frombuffer(bytes(m), np.uint8) if len(m) else np.zeros(1, np.uint8) for m in msgs]
+        out_bytes = [0] * n
+        self.out = []
+        self._sigs, self._seeds = [], []
+        for i in range(n):
+            op, mode = int(ops[i]), int(modes[i])
+            pkb, skb, sigb = sizes(mode)
+            r = self.reqs[i]
+            r.op, r.mode, r.key = op, mode, int(keys[i])
+            r.mlen = len(msgs[i])
+            r.msg = self._msgs[i].ctypes.data
+            if op == OP_KEYGEN:
+                sd = np.ascontiguousarray(seeds[i], dtype=np.uint8)
+                self._seeds.append(sd)
+                r.seed = sd.ctypes.data
+                o = np.zeros(pkb + skb, np.uint8)
+            elif op == OP_SIGN:
+                o = np.zeros(sigb, np.uint8)
+            else:
+                sg = np.ascontiguousarray(sigs[i], dtype=np.uint8)
+                self._sigs.append(sg)
+                r.sig = sg.ctypes.data
+                o = np.zeros(1, np.uint8)
+            self.out.append(o)
+            r.out = o.ctypes.data
+
+    def field(self, name) -> np.ndarray:
+        return np.array([getattr(r, name) for r in self.reqs])

@@ -304,34 +304,91 @@ struct dil_sign_ctx {
     // timing: one (start, fixed done, loop done) event triple per timed call
     std::vector<hipEvent_t> ev;
     size_t timing_cap = 0, timed = 0;
+    bool resident_only = false;  // rows without the per-item expanded arrays
 };
 
-extern "C" {
+// Resident keys: KeyLayout rows (pk, sk, tr, A, s1^, s2^, t0^, t1^).
+struct dil_keystore {
+    int mode = 0;
+    size_t capacity = 0, pitch = 0;
+    uint8_t *rows = nullptr;
+};
 
-int dil_sign_ctx_destroy(dil_sign_ctx *c) {
-    if (!c) return 0;
-    (void)hipFree(c->rows);
-    (void)hipFree(c->slots);
-    (void)hipFree(c->d_head);
-    (void)hipFree(c->d_exec);
-    (void)hipFree(c->d_done);
-    (void)hipFree(c->d_copy);
-    (void)hipHostFree(c->h_exec);
-    (void)hipHostFree(c->h_done);
-    (void)hipHostFree(c->h_copy);
-    for (auto &e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    delete c;
-    return 0;
+// sign n items; with ks, item i uses resident key d_key_idx[i] (d_sk unused)
+static int sign_enqueue(dil_sign_ctx *c, uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_sk, size_t sk_pitch,
+                        const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, int engine,
+                        hipStream_t st, bool timing, const dil_keystore *ks = nullptr,
+                        const uint32_t *d_key_idx = nullptr) {
+    if (!c) return DIL_EINVAL;
+    if (n == 0) return 0;
+    if (n > c->max_batch) return DIL_ERANGE;
+    if (ks) {
+        if (ks->mode != c->mode || !d_key_idx || ks->capacity == 0) return DIL_EINVAL;
+        d_sk = ks->rows + with_params(c->mode, [](auto p) { return KeyLayout<decltype(p)>::SK; });
+        sk_pitch = ks->pitch;
+    } else if (c->resident_only) {
+        return DIL_EINVAL;
+    }
+    if (!d_sig || !d_sk || !d_msg || !d_moff || !d_mlen) return DIL_EINVAL;
+    if (sig_pitch < dil_signature_bytes(c->mode) || sk_pitch < dil_secret_key_bytes(c->mode)) return DIL_EINVAL;
+    if (engine != DIL_ENGINE_PERSISTENT && engine != DIL_ENGINE_PERSTAGE) return DIL_EINVAL;
+    if (engine == DIL_ENGINE_PERSTAGE && n > 65536) return DIL_ERANGE;
+    return with_params(c->mode, [&](auto p) -> int {
+        using P = decltype(p);
+        SignArgs a{};
+        a.sig = d_sig;
+        a.sig_pitch = sig_pitch;
+        a.sk = d_sk;
+        a.sk_pitch = sk_pitch;
+        a.m = MsgRef{d_msg, d_moff, 0, d_mlen, 0};
+        a.rows = c->rows;
+        a.row_pitch = c->row_pitch;
+        a.off_ints = SignLayout<P>::ints(0);
+        a.n = n;
+        if (ks) {
+            a.key_idx = d_key_idx;
+            a.key_ints = ks->rows + KeyLayout<P>::MAT;
+            a.key_pitch = ks->pitch;
+            a.nkeys = (uint32_t)ks->capacity;
+        }
+        const SlotRef slots{c->slots + TempLayout<P>::SLOT, c->slot_pitch, c->slots + TempLayout<P>::SIG,
+                            c->slot_pitch};
+        int e;
+        hipEvent_t *e3 = timing && c->timed < c->timing_cap ? &c->ev[3 * c->timed] : nullptr;
+        if (e3 && (e = (int)hipEventRecord(e3[0], st))) return e;
+        if (engine == DIL_ENGINE_PERSTAGE) {
+            // the per-stage scheduler launches the fixed part itself
+            if (e3 && (e = (int)hipEventRecord(e3[1], st))) return e;
+            const SlotRef items{c->rows + SignLayout<P>::SLOT, c->row_pitch, d_sig, sig_pitch};
+            e = sign_perstage(c->mode, a, items, slots, c->nslots, c->sign_lut.data(), c->h_exec, c->d_exec,
+                              c->h_done, c->d_done, c->h_copy, c->d_copy, st);
+        } else {
+            if ((e = launch_sign_fixed(c->mode, a, st))) return e;
+            if (e3 && (e = (int)hipEventRecord(e3[1], st))) return e;
+            const int spb = c->slots_per_block ? c->slots_per_block : default_slots_per_block(c->nslots);
+            e = launch_sign_persistent(c->mode, a, slots, c->nslots, spb, c->occupancy, c->blocks, c->d_head, st);
+            if (e == -3) return DIL_ERANGE;
+        }
+        if (e) return e;
+        if (e3) {
+            if ((e = (int)hipEventRecord(e3[2], st))) return e;
+            ++c->timed;
+        }
+        return 0;
+    });
 }
 
-int dil_sign_ctx_create(dil_sign_ctx **out, int mode, size_t max_batch, size_t nslots) {
+
+static int ctx_create(dil_sign_ctx **out, int mode, size_t max_batch, size_t nslots, bool resident_only) {
     if (!out || !valid_mode(mode) || max_batch == 0 || nslots == 0) return DIL_EINVAL;
     dil_sign_ctx *c = new dil_sign_ctx();
     c->mode = mode;
     c->max_batch = max_batch;
     c->nslots = nslots;
-    c->row_pitch = align256(dil_sign_pool_row_bytes(mode, 0));
+    c->resident_only = resident_only;
+    c->row_pitch = resident_only
+                       ? with_params(mode, [](auto p) { return align256(SignLayout<decltype(p)>::ints(0)); })
+                       : align256(dil_sign_pool_row_bytes(mode, 0));
     c->slot_pitch = align256(dil_sign_temp_row_bytes(mode));
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t r) {
@@ -353,6 +410,33 @@ int dil_sign_ctx_create(dil_sign_ctx **out, int mode, size_t max_batch, size_t n
     }
     *out = c;
     return 0;
+}
+
+extern "C" {
+
+int dil_sign_ctx_destroy(dil_sign_ctx *c) {
+    if (!c) return 0;
+    (void)hipFree(c->rows);
+    (void)hipFree(c->slots);
+    (void)hipFree(c->d_head);
+    (void)hipFree(c->d_exec);
+    (void)hipFree(c->d_done);
+    (void)hipFree(c->d_copy);
+    (void)hipHostFree(c->h_exec);
+    (void)hipHostFree(c->h_done);
+    (void)hipHostFree(c->h_copy);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete c;
+    return 0;
+}
+
+int dil_sign_ctx_create(dil_sign_ctx **out, int mode, size_t max_batch, size_t nslots) {
+    return ctx_create(out, mode, max_batch, nslots, false);
+}
+
+int dil_sign_ctx_create_resident(dil_sign_ctx **out, int mode, size_t max_batch, size_t nslots) {
+    return ctx_create(out, mode, max_batch, nslots, true);
 }
 
 int dil_sign_ctx_set_tiling(dil_sign_ctx *c, int slots_per_block, int occupancy, int blocks) {
@@ -423,51 +507,220 @@ int dil_sign_ctx_attempt_counts(dil_sign_ctx *c, uint32_t *h_counts, size_t n, v
 int dil_sign_device(dil_sign_ctx *c, uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_sk, size_t sk_pitch,
                     const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, int engine,
                     void *stream) {
-    if (!c) return DIL_EINVAL;
+    return sign_enqueue(c, d_sig, sig_pitch, d_sk, sk_pitch, d_msg, d_moff, d_mlen, n, engine, S(stream), true);
+}
+
+}  // extern "C"
+
+/* ------------------------------------------------------------ HIP graphs */
+
+// A captured pipeline: every launch of one keygen / sign / verify call with
+// its arguments fixed, replayed by one hipGraphLaunch.  Capture runs on a
+// private stream in thread-local mode, so other threads' HIP calls do not
+// interfere; the fixed sign stage's fork/join over the auxiliary streams
+// becomes parallel branches of the graph.  No memset nodes are recorded
+// (control words are cleared by a kernel).
+struct dil_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t n = 0;
+};
+
+extern "C" int dil_graph_destroy(dil_graph *g) {
+    if (!g) return 0;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return 0;
+}
+
+namespace {
+
+template <class F>
+int capture_graph(dil_graph **out, size_t n, F &&enqueue) {
+    if (!out) return DIL_EINVAL;
+    *out = nullptr;
+    hipStream_t cap = nullptr;
+    int e = (int)hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (e) return e;
+    dil_graph *g = new dil_graph();
+    g->n = n;
+    e = (int)hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (!e) {
+        const int le = enqueue(cap);
+        hipGraph_t gr = nullptr;
+        const int ee = (int)hipStreamEndCapture(cap, &gr);
+        g->graph = gr;
+        e = le ? le : ee;
+    }
+    if (!e) e = (int)hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cap);
+    if (e) {
+        dil_graph_destroy(g);
+        return e;
+    }
+    *out = g;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dil_sign_graph_create(dil_graph **g, dil_sign_ctx *c, uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_sk,
+                          size_t sk_pitch, const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen,
+                          size_t n) {
+    if (!c || n == 0) return DIL_EINVAL;
+    return capture_graph(g, n, [&](hipStream_t st) {
+        return sign_enqueue(c, d_sig, sig_pitch, d_sk, sk_pitch, d_msg, d_moff, d_mlen, n, DIL_ENGINE_PERSISTENT,
+                            st, false);
+    });
+}
+
+int dil_verify_graph_create(dil_graph **g, int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch,
+                            const uint8_t *d_pk, size_t pk_pitch, const uint8_t *d_msg, const uint64_t *d_moff,
+                            const uint32_t *d_mlen, size_t n, uint8_t *d_work, size_t work_pitch) {
+    if (n == 0) return DIL_EINVAL;
+    return capture_graph(g, n, [&](hipStream_t st) {
+        return dil_verify_device(mode, d_ret, d_sig, sig_pitch, d_pk, pk_pitch, d_msg, d_moff, d_mlen, n, d_work,
+                                 work_pitch, st);
+    });
+}
+
+int dil_keypair_graph_create(dil_graph **g, int mode, uint8_t *d_pk, size_t pk_pitch, uint8_t *d_sk,
+                             size_t sk_pitch, const uint8_t *d_seeds, size_t seed_pitch, uint64_t rand_index,
+                             size_t n, uint8_t *d_work, size_t work_pitch) {
+    if (n == 0) return DIL_EINVAL;
+    return capture_graph(g, n, [&](hipStream_t st) {
+        return dil_keypair_device(mode, d_pk, pk_pitch, d_sk, sk_pitch, d_seeds, seed_pitch, rand_index, n, d_work,
+                                  work_pitch, st);
+    });
+}
+
+int dil_graph_launch(dil_graph *g, void *stream) {
+    if (!g || !g->exec) return DIL_EINVAL;
+    return (int)hipGraphLaunch(g->exec, S(stream));
+}
+
+size_t dil_graph_batch(const dil_graph *g) { return g ? g->n : 0; }
+
+}  // extern "C"
+
+/* ----------------------------------------------------------- key stores */
+
+extern "C" {
+
+size_t dil_keystore_row_bytes(int mode) {
+    if (!valid_mode(mode)) return 0;
+    return with_params(mode, [](auto p) { return KeyLayout<decltype(p)>::ROW; });
+}
+
+int dil_keystore_create(dil_keystore **out, int mode, size_t capacity) {
+    if (!out || !valid_mode(mode) || capacity == 0 || capacity > 0xFFFFFFFFull) return DIL_EINVAL;
+    dil_keystore *k = new dil_keystore();
+    k->mode = mode;
+    k->capacity = capacity;
+    k->pitch = align256(dil_keystore_row_bytes(mode));
+    const int e = (int)hipMalloc((void **)&k->rows, k->pitch * capacity);
+    if (e) {
+        delete k;
+        return e;
+    }
+    *out = k;
+    return 0;
+}
+
+int dil_keystore_destroy(dil_keystore *k) {
+    if (!k) return 0;
+    (void)hipFree(k->rows);
+    delete k;
+    return 0;
+}
+
+size_t dil_keystore_capacity(const dil_keystore *k) { return k ? k->capacity : 0; }
+
+int dil_keystore_mode(const dil_keystore *k) { return k ? k->mode : 0; }
+
+int dil_keystore_load(dil_keystore *k, size_t first, const uint8_t *pk, size_t pk_pitch, const uint8_t *sk,
+                      size_t sk_pitch, size_t n, void *stream) {
+    if (!k || !pk) return DIL_EINVAL;
     if (n == 0) return 0;
-    if (n > c->max_batch) return DIL_ERANGE;
-    if (!d_sig || !d_sk || !d_msg || !d_moff || !d_mlen) return DIL_EINVAL;
-    if (sig_pitch < dil_signature_bytes(c->mode) || sk_pitch < dil_secret_key_bytes(c->mode)) return DIL_EINVAL;
-    if (engine != DIL_ENGINE_PERSISTENT && engine != DIL_ENGINE_PERSTAGE) return DIL_EINVAL;
-    if (engine == DIL_ENGINE_PERSTAGE && n > 65536) return DIL_ERANGE;
+    if (first > k->capacity || n > k->capacity - first) return DIL_ERANGE;
+    if (pk_pitch < dil_public_key_bytes(k->mode) || (sk && sk_pitch < dil_secret_key_bytes(k->mode)))
+        return DIL_EINVAL;
     const hipStream_t st = S(stream);
-    return with_params(c->mode, [&](auto p) -> int {
+    return with_params(k->mode, [&](auto p) -> int {
         using P = decltype(p);
-        SignArgs a{};
-        a.sig = d_sig;
-        a.sig_pitch = sig_pitch;
-        a.sk = d_sk;
-        a.sk_pitch = sk_pitch;
-        a.m = MsgRef{d_msg, d_moff, 0, d_mlen, 0};
-        a.rows = c->rows;
-        a.row_pitch = c->row_pitch;
-        a.off_ints = SignLayout<P>::ints(0);
-        a.n = n;
-        const SlotRef slots{c->slots + TempLayout<P>::SLOT, c->slot_pitch, c->slots + TempLayout<P>::SIG,
-                            c->slot_pitch};
+        using Z = Sizes<P>;
+        using KY = KeyLayout<P>;
+        uint8_t *row = k->rows + first * k->pitch;
         int e;
-        hipEvent_t *e3 = c->timed < c->timing_cap ? &c->ev[3 * c->timed] : nullptr;
-        if (e3 && (e = (int)hipEventRecord(e3[0], st))) return e;
-        if (engine == DIL_ENGINE_PERSTAGE) {
-            // the per-stage scheduler launches the fixed part itself
-            if (e3 && (e = (int)hipEventRecord(e3[1], st))) return e;
-            const SlotRef items{c->rows + SignLayout<P>::SLOT, c->row_pitch, d_sig, sig_pitch};
-            e = sign_perstage(c->mode, a, items, slots, c->nslots, c->sign_lut.data(), c->h_exec, c->d_exec,
-                              c->h_done, c->d_done, c->h_copy, c->d_copy, st);
-        } else {
-            if ((e = launch_sign_fixed(c->mode, a, st))) return e;
-            if (e3 && (e = (int)hipEventRecord(e3[1], st))) return e;
-            const int spb = c->slots_per_block ? c->slots_per_block : default_slots_per_block(c->nslots);
-            e = launch_sign_persistent(c->mode, a, slots, c->nslots, spb, c->occupancy, c->blocks, c->d_head, st);
-            if (e == -3) return DIL_ERANGE;
-        }
-        if (e) return e;
-        if (e3) {
-            if ((e = (int)hipEventRecord(e3[2], st))) return e;
-            ++c->timed;
-        }
+        if ((e = copy2d(row + KY::PK, k->pitch, pk, pk_pitch, Z::PK_BYTES, n, hipMemcpyDefault, st))) return e;
+        if (sk && (e = copy2d(row + KY::SK, k->pitch, sk, sk_pitch, Z::SK_BYTES, n, hipMemcpyDefault, st)))
+            return e;
+        return launch_key_expand(k->mode, row, k->pitch, n, sk ? 1 : 0, st);
+    });
+}
+
+int dil_keystore_keygen(dil_keystore *k, size_t first, const uint8_t *d_seeds, size_t seed_pitch,
+                        uint64_t rand_index, size_t n, void *stream) {
+    if (!k) return DIL_EINVAL;
+    if (n == 0) return 0;
+    if (first > k->capacity || n > k->capacity - first) return DIL_ERANGE;
+    if (d_seeds && seed_pitch < 32) return DIL_EINVAL;
+    const hipStream_t st = S(stream);
+    return with_params(k->mode, [&](auto p) -> int {
+        using KY = KeyLayout<decltype(p)>;
+        uint8_t *row = k->rows + first * k->pitch;
+        // the int area is keygen's scratch until the expansion overwrites it
+        KeygenArgs a{row + KY::PK, k->pitch, row + KY::SK, k->pitch, row + KY::INTS, k->pitch,
+                     d_seeds, seed_pitch, rand_index, n};
+        int e = launch_keygen(k->mode, a, st);
+        return e ? e : launch_key_expand(k->mode, row, k->pitch, n, 1, st);
+    });
+}
+
+int dil_keystore_export(const dil_keystore *k, size_t first, size_t n, uint8_t *pk, size_t pk_pitch, uint8_t *sk,
+                        size_t sk_pitch, void *stream) {
+    if (!k) return DIL_EINVAL;
+    if (n == 0) return 0;
+    if (first > k->capacity || n > k->capacity - first) return DIL_ERANGE;
+    if ((pk && pk_pitch < dil_public_key_bytes(k->mode)) || (sk && sk_pitch < dil_secret_key_bytes(k->mode)))
+        return DIL_EINVAL;
+    const hipStream_t st = S(stream);
+    return with_params(k->mode, [&](auto p) -> int {
+        using P = decltype(p);
+        using Z = Sizes<P>;
+        using KY = KeyLayout<P>;
+        const uint8_t *row = k->rows + first * k->pitch;
+        int e;
+        if (pk && (e = copy2d(pk, pk_pitch, row + KY::PK, k->pitch, Z::PK_BYTES, n, hipMemcpyDefault, st))) return e;
+        if (sk && (e = copy2d(sk, sk_pitch, row + KY::SK, k->pitch, Z::SK_BYTES, n, hipMemcpyDefault, st))) return e;
         return 0;
     });
+}
+
+int dil_sign_device_resident(dil_sign_ctx *c, const dil_keystore *k, const uint32_t *d_key_idx, uint8_t *d_sig,
+                             size_t sig_pitch, const uint8_t *d_msg, const uint64_t *d_moff,
+                             const uint32_t *d_mlen, size_t n, int engine, void *stream) {
+    if (!k) return DIL_EINVAL;
+    return sign_enqueue(c, d_sig, sig_pitch, nullptr, 0, d_msg, d_moff, d_mlen, n, engine, S(stream), true, k,
+                        d_key_idx);
+}
+
+int dil_verify_device_resident(const dil_keystore *k, const uint32_t *d_key_idx, int32_t *d_ret,
+                               const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_msg,
+                               const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, uint8_t *d_work,
+                               size_t work_pitch, void *stream) {
+    if (!k || !d_key_idx) return DIL_EINVAL;
+    if (n == 0) return 0;
+    const int mode = k->mode;
+    if (!d_ret || !d_sig || !d_work || !d_moff || !d_mlen || !d_msg) return DIL_EINVAL;
+    if (sig_pitch < dil_signature_bytes(mode)) return DIL_EINVAL;
+    if (work_pitch < dil_verify_work_row_bytes(mode) || (work_pitch & 255)) return DIL_ENOMEM;
+    VerifyArgs a{d_sig, sig_pitch, k->rows, k->pitch, MsgRef{d_msg, d_moff, 0, d_mlen, 0}, d_work, work_pitch,
+                 d_ret, 1, n, d_key_idx, k->rows, k->pitch, (uint32_t)k->capacity};
+    return launch_verify(mode, a, S(stream));
 }
 
 }  // extern "C"

@@ -97,4 +97,28 @@ struct VerifyLayout {
     }
 };
 
+
+// One resident key of a key store: public key, secret key (absent for
+// verify-only keys), tr = H(pk), and the expanded arrays every signature and
+// verification under the key would otherwise recompute: A = ExpandA(rho)
+// (K*L polys, 5*K*L Keccak permutations), s1^, s2^, t0^ (NTT domain) and
+// t1^ = NTT(t1 * 2^d), int32 in the NTT kernels' register layout.
+template <class P>
+struct KeyLayout {
+    using Z = Sizes<P>;
+    static constexpr size_t PK = 0;
+    static constexpr size_t SK = align256(Z::PK_BYTES);
+    static constexpr size_t TR = SK + align256(Z::SK_BYTES);  // tr (32) | flags (4)
+    static constexpr size_t FLAGS = TR + SEEDBYTES;
+    static constexpr size_t INTS = TR + 256;
+    static constexpr size_t MAT = INTS;
+    static constexpr size_t S1H = MAT + (size_t)P::K * P::L * N * 4;
+    static constexpr size_t S2H = S1H + (size_t)P::L * N * 4;
+    static constexpr size_t T0H = S2H + (size_t)P::K * N * 4;
+    static constexpr size_t T1H = T0H + (size_t)P::K * N * 4;
+    static constexpr size_t ROW = T1H + (size_t)P::K * N * 4;
+    // keygen into the store uses the (not yet expanded) int area as scratch
+    static_assert(KeypairLayout<P>::ROW - KeypairLayout<P>::SCRATCH <= ROW - INTS, "keygen scratch fits");
+};
+
 }  // namespace dil

@@ -46,6 +46,14 @@ struct VerifyArgs {
     int32_t *ret;
     size_t ret_pitch;  // in int32 units
     size_t n;
+    // Resident keys (optional): item i verifies under key key_idx[i] of the
+    // store at keys (nkeys KeyLayout rows of key_pitch bytes); pk / pk_pitch
+    // are then ignored and tr, A and t1^ come from the store.  An index
+    // >= nkeys rejects the item (ret -1) without touching the store.
+    const uint32_t *key_idx;
+    const uint8_t *keys;
+    size_t key_pitch;
+    uint32_t nkeys;
 };
 
 // Host-side per-item state of the per-stage scheduler, laid out as the
@@ -59,6 +67,9 @@ struct SignArgs;
 struct SlotRef;
 
 int launch_keygen(int mode, const KeygenArgs &a, hipStream_t st);
+// Expands keys [0, n) of a key store (KeyLayout rows from `keys`): tr, A and
+// t1^ from the public key, and s1^, s2^, t0^ when with_sk.
+int launch_key_expand(int mode, uint8_t *keys, size_t key_pitch, size_t n, int with_sk, hipStream_t st);
 int launch_verify(int mode, const VerifyArgs &a, hipStream_t st);
 int launch_sign_fixed(int mode, const SignArgs &a, hipStream_t st);
 int launch_sign_round(int mode, const SignArgs &a, const SlotRef &sr, const uint32_t *d_exec, size_t nslot,

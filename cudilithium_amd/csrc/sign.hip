@@ -123,6 +123,11 @@ template <class P>
 int sign_fixed_impl(const SignArgs &a, hipStream_t st) {
     if (a.n == 0) return 0;
     constexpr int KL = P::K * P::L, NP = P::L + 2 * P::K;
+    if (a.key_idx) {
+        // resident keys carry A, s1^, s2^, t0^ already: only mu / rhoprime
+        hipLaunchKernelGGL(sg_mu_kernel<P>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a);
+        return (int)hipGetLastError();
+    }
     AuxStreams *aux = aux_streams();
     if (!aux) return (int)hipErrorOutOfMemory;
     std::lock_guard<std::mutex> guard(aux->lock);

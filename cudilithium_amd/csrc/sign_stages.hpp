@@ -33,7 +33,22 @@ struct SignArgs {
     size_t row_pitch;
     size_t off_ints;  // byte offset of the int32 arrays (A, s1^, s2^, t0^) in a row
     size_t n;
+    // Resident keys (optional): when key_idx is set, item i signs with key
+    // k = min(key_idx[i], nkeys - 1); its secret key is sk + k * sk_pitch and
+    // its expanded arrays (A, s1^, s2^, t0^) sit at key_ints + k * key_pitch
+    // instead of in the item's row.  (Out-of-range indices are clamped so
+    // that no access leaves the store.)
+    const uint32_t *key_idx;
+    const uint8_t *key_ints;
+    size_t key_pitch;
+    uint32_t nkeys;
 };
+
+__device__ __forceinline__ size_t sk_index(const SignArgs &a, size_t i) {
+    if (!a.key_idx) return i;
+    const uint32_t k = a.key_idx[i];
+    return k < a.nkeys ? k : a.nkeys - 1;
+}
 
 // Scratch and signature destination of attempt slots.
 struct SlotRef {
@@ -51,7 +66,8 @@ struct ItemView {
     const int32_t *mat, *s1h, *s2h, *t0h;
     __device__ ItemView(const SignArgs &a, size_t item) {
         row = a.rows + item * a.row_pitch;
-        mat = reinterpret_cast<const int32_t *>(row + a.off_ints);
+        mat = a.key_idx ? reinterpret_cast<const int32_t *>(a.key_ints + sk_index(a, item) * a.key_pitch)
+                        : reinterpret_cast<const int32_t *>(row + a.off_ints);
         s1h = mat + P::K * P::L * N;
         s2h = s1h + P::L * N;
         t0h = s2h + P::K * N;
@@ -69,7 +85,7 @@ template <class P>
 __device__ __forceinline__ void stage_mu(const SignArgs &a, size_t i) {
     using Z = Sizes<P>;
     using SLy = SignLayout<P>;
-    const uint8_t *sk = a.sk + i * a.sk_pitch;
+    const uint8_t *sk = a.sk + sk_index(a, i) * a.sk_pitch;
     uint64_t tr[4], key[4];
 #pragma unroll
     for (int w = 0; w < 4; ++w) {

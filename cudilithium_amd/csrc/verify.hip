@@ -28,9 +28,14 @@ __global__ __launch_bounds__(256) void vf_prep_kernel(VerifyArgs a) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const uint8_t *sig = a.sig + i * a.sig_pitch;
-    const uint8_t *pk = a.pk + i * a.pk_pitch;
     uint8_t *work = a.work + i * a.work_pitch;
     int32_t *flag = reinterpret_cast<int32_t *>(work + VL::FLAG);
+    if (a.key_idx && a.key_idx[i] >= a.nkeys) {
+        *flag = 1;
+        return;
+    }
+    const uint8_t *key = a.key_idx ? a.keys + a.key_idx[i] * a.key_pitch : nullptr;
+    const uint8_t *pk = key ? key + KeyLayout<P>::PK : a.pk + i * a.pk_pitch;
     uint32_t *bm = reinterpret_cast<uint32_t *>(work + VL::HINT);
 
     // hint decoding rules (verify.cu:52-93; spec unpack_sig)
@@ -60,13 +65,19 @@ __global__ __launch_bounds__(256) void vf_prep_kernel(VerifyArgs a) {
     *flag = bad;
     if (bad) return;
 
-    // tr = SHAKE256(pk)[0:32]; mu = SHAKE256(tr || m)[0:64] (verify.cu:130-144)
+    // tr = SHAKE256(pk)[0:32] (stored with a resident key); mu = SHAKE256(tr || m)[0:64]
+    // (verify.cu:130-144)
     Lane64 s[25];
-    shake_absorb<SHAKE256_RATE>(s, [&](uint32_t w, uint32_t nb) {
-        return nb == 8 ? load_le64(pk + 8 * w) : load_le_partial(pk + 8 * w, nb);
-    }, (uint32_t)Z::PK_BYTES);
     uint64_t tr[4];
-    squeeze_words<4>(s, tr);
+    if (key) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) tr[w] = load_le64(key + KeyLayout<P>::TR + 8 * w);
+    } else {
+        shake_absorb<SHAKE256_RATE>(s, [&](uint32_t w, uint32_t nb) {
+            return nb == 8 ? load_le64(pk + 8 * w) : load_le_partial(pk + 8 * w, nb);
+        }, (uint32_t)Z::PK_BYTES);
+        squeeze_words<4>(s, tr);
+    }
     const uint8_t *m = a.m.ptr(i);
     const uint32_t mlen = a.m.len(i);
     shake_absorb<SHAKE256_RATE>(s, [&](uint32_t w, uint32_t nb) -> uint64_t {
@@ -91,28 +102,31 @@ struct VfMainCfg {
     static constexpr int G = P::MODE == 2 ? 16 : (P::MODE == 3 ? 8 : 4);
 };
 
-template <class P>
+// RES: keys are resident (A and t1^ read from the store; no accumulator)
+template <class P, bool RES>
 struct VfShared {
     static constexpr int G = VfMainCfg<P>::G;
     ZetaLds zt;
     int32_t x[4][NTT_XBUF];
     int32_t zhat[G][P::L][N];
-    int32_t acc[G][P::K][N];
+    int32_t acc[RES ? 1 : G][P::K][N];
     int32_t chat[G][N];
     int32_t fail[G];
 };
 
-template <class P>
+template <class P, bool RES>
 __global__ __launch_bounds__(256) void vf_main_kernel(VerifyArgs a) {
     using Z = Sizes<P>;
     using VL = VerifyLayout<P>;
+    using KL = KeyLayout<P>;
     constexpr int G = VfMainCfg<P>::G;
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    VfShared<P> &sh = *reinterpret_cast<VfShared<P> *>(smem_raw);
+    VfShared<P, RES> &sh = *reinterpret_cast<VfShared<P, RES> *>(smem_raw);
     const size_t item0 = (size_t)blockIdx.x * G;
     const int wave = threadIdx.x >> 6;
     zeta_lds_load(sh.zt);
-    for (int j = threadIdx.x; j < G * P::K * N; j += blockDim.x) (&sh.acc[0][0][0])[j] = 0;
+    if constexpr (!RES)
+        for (int j = threadIdx.x; j < G * P::K * N; j += blockDim.x) (&sh.acc[0][0][0])[j] = 0;
     if (threadIdx.x < G) {
         const size_t item = item0 + threadIdx.x;
         sh.fail[threadIdx.x] =
@@ -150,6 +164,7 @@ __global__ __launch_bounds__(256) void vf_main_kernel(VerifyArgs a) {
     __syncthreads();
 
     // A z^ with A streamed from its sponges
+    if constexpr (!RES)
     for (int t = threadIdx.x; t < G * P::K * P::L; t += blockDim.x) {
         const int g = t / (P::K * P::L), kl = t % (P::K * P::L);
         const int k = kl / P::L, l = kl % P::L;
@@ -172,13 +187,29 @@ __global__ __launch_bounds__(256) void vf_main_kernel(VerifyArgs a) {
         const size_t item = item0 + g;
         if (item >= a.n || sh.fail[g]) continue;
         uint8_t *work = a.work + item * a.work_pitch;
-        uint32_t f[4];
-        wave_fields_A<10>(f, a.pk + item * a.pk_pitch + SEEDBYTES + k * POLYT1_BYTES);
         int32_t t1[4], w[4], c[4];
+        if constexpr (RES) {
+            // A z^ from the stored A, in the order of the accumulating form (exact int32 sums)
+            const uint8_t *key = a.keys + a.key_idx[item] * a.key_pitch;
+            const int32_t *mat = reinterpret_cast<const int32_t *>(key + KL::MAT);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) t1[r] = (int32_t)(f[r] << D);
-        ntt_wave(t1, sh.x[wave], sh.zt);
-        wave_load_D(w, sh.acc[g][k]);
+            for (int r = 0; r < 4; ++r) w[r] = 0;
+            for (int l = 0; l < P::L; ++l) {
+                int32_t av[4], zv[4];
+                wave_load_D(av, mat + (k * P::L + l) * N);
+                wave_load_D(zv, sh.zhat[g][l]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) w[r] += mont_mul(av[r], zv[r]);
+            }
+            wave_load_D(t1, reinterpret_cast<const int32_t *>(key + KL::T1H) + k * N);
+        } else {
+            uint32_t f[4];
+            wave_fields_A<10>(f, a.pk + item * a.pk_pitch + SEEDBYTES + k * POLYT1_BYTES);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) t1[r] = (int32_t)(f[r] << D);
+            ntt_wave(t1, sh.x[wave], sh.zt);
+            wave_load_D(w, sh.acc[g][k]);
+        }
         wave_load_D(c, sh.chat[g]);
 #pragma unroll
         for (int r = 0; r < 4; ++r) w[r] = reduce32(w[r] - mont_mul(c[r], t1[r]));
@@ -223,17 +254,24 @@ __global__ __launch_bounds__(256) void vf_final_kernel(VerifyArgs a) {
     a.ret[i * a.ret_pitch] = verdict;
 }
 
+template <class P, bool RES>
+static int launch_vf_main(const VerifyArgs &a, hipStream_t st) {
+    constexpr int G = VfMainCfg<P>::G;
+    const size_t smem = sizeof(VfShared<P, RES>);
+    hipError_t e = hipFuncSetAttribute((const void *)vf_main_kernel<P, RES>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL((vf_main_kernel<P, RES>), dim3((unsigned)((a.n + G - 1) / G)), dim3(256), smem, st, a);
+    return 0;
+}
+
 template <class P>
 static int verify_impl(const VerifyArgs &a, hipStream_t st) {
     if (a.n == 0) return 0;
-    constexpr int G = VfMainCfg<P>::G;
     const unsigned b1 = (unsigned)((a.n + 255) / 256);
     hipLaunchKernelGGL(vf_prep_kernel<P>, dim3(b1), dim3(256), 0, st, a);
-    const size_t smem = sizeof(VfShared<P>);
-    hipError_t e = hipFuncSetAttribute((const void *)vf_main_kernel<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)smem);
-    if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL(vf_main_kernel<P>, dim3((unsigned)((a.n + G - 1) / G)), dim3(256), smem, st, a);
+    const int e = a.key_idx ? launch_vf_main<P, true>(a, st) : launch_vf_main<P, false>(a, st);
+    if (e) return e;
     hipLaunchKernelGGL(vf_final_kernel<P>, dim3(b1), dim3(256), 0, st, a);
     return (int)hipGetLastError();
 }

@@ -222,6 +222,141 @@ int dil_sign_ctx_status(dil_sign_ctx *ctx, int *status, void *stream);
  * included (synchronises `stream`). */
 int dil_sign_ctx_executed_attempts(dil_sign_ctx *ctx, uint64_t *count, void *stream);
 
+/* Resident keys (no counterpart in the reference, which re-derives A from
+ * rho on every signature and verification, sign.cu:98-163 and
+ * verify.cu:146-200, and re-unpacks and transforms s1, s2, t0 on every
+ * signature, sign.cu:306-403).  A key store holds `capacity` keys of one
+ * mode: pk, sk (absent for verify-only keys), tr = H(pk) and the expanded
+ * A, s1^, s2^, t0^, t1^ (dil_keystore_row_bytes(mode) each, about 38/60/95
+ * KiB for modes 2/3/5).  Results equal the non-resident entry points'. */
+typedef struct dil_keystore dil_keystore;
+size_t dil_keystore_row_bytes(int mode);
+int dil_keystore_create(dil_keystore **ks, int mode, size_t capacity);
+int dil_keystore_destroy(dil_keystore *ks);
+size_t dil_keystore_capacity(const dil_keystore *ks);
+int dil_keystore_mode(const dil_keystore *ks);
+/* Installs n keys at [first, first + n) from pk (and sk, or NULL for
+ * verify-only keys), host or device memory, and expands them on `stream`. */
+int dil_keystore_load(dil_keystore *ks, size_t first, const uint8_t *pk, size_t pk_pitch,
+                      const uint8_t *sk, size_t sk_pitch, size_t n, void *stream);
+/* Generates n keys into [first, first + n) as dil_keypair_device would
+ * (explicit seeds, or counter seeds from rand_index when d_seeds is NULL)
+ * and expands them. */
+int dil_keystore_keygen(dil_keystore *ks, size_t first, const uint8_t *d_seeds, size_t seed_pitch,
+                        uint64_t rand_index, size_t n, void *stream);
+/* Copies pk and/or sk (either may be NULL) of keys [first, first + n) out. */
+int dil_keystore_export(const dil_keystore *ks, size_t first, size_t n, uint8_t *pk, size_t pk_pitch,
+                        uint8_t *sk, size_t sk_pitch, void *stream);
+/* A signing context whose rows omit the per-item expanded arrays; it only
+ * accepts dil_sign_device_resident. */
+int dil_sign_ctx_create_resident(dil_sign_ctx **ctx, int mode, size_t max_batch, size_t nslots);
+/* dil_sign_device with item i signed by resident key d_key_idx[i] (device
+ * array; keys must have been loaded with their secret key).  Indices must
+ * be below the store's capacity (larger ones are clamped to the last key). */
+int dil_sign_device_resident(dil_sign_ctx *ctx, const dil_keystore *ks, const uint32_t *d_key_idx,
+                             uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_msg,
+                             const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, int engine,
+                             void *stream);
+/* dil_verify_device with item i verified under resident key d_key_idx[i]; an
+ * index at or above the capacity gives ret -1. */
+int dil_verify_device_resident(const dil_keystore *ks, const uint32_t *d_key_idx, int32_t *d_ret,
+                               const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_msg,
+                               const uint64_t *d_moff, const uint32_t *d_mlen, size_t n,
+                               uint8_t *d_work, size_t work_pitch, void *stream);
+
+/* HIP graphs of whole pipelines (no counterpart in the reference, whose
+ * host scheduler synchronises once per rejection round, api.cu:208-441).
+ * Each *_graph_create captures every launch of one call of the matching
+ * *_device function, with these exact arguments, into an instantiated
+ * graph; dil_graph_launch replays it on `stream` (one hipGraphLaunch).
+ * Buffers named at creation must stay allocated while the graph exists;
+ * replays of graphs sharing buffers (the same sign context, work rows or
+ * outputs) must not overlap in time.  Sign graphs use the persistent
+ * engine with the context's tiling at creation. */
+typedef struct dil_graph dil_graph;
+int dil_sign_graph_create(dil_graph **graph, dil_sign_ctx *ctx, uint8_t *d_sig, size_t sig_pitch,
+                          const uint8_t *d_sk, size_t sk_pitch, const uint8_t *d_msg,
+                          const uint64_t *d_moff, const uint32_t *d_mlen, size_t n);
+int dil_verify_graph_create(dil_graph **graph, int mode, int32_t *d_ret, const uint8_t *d_sig,
+                            size_t sig_pitch, const uint8_t *d_pk, size_t pk_pitch,
+                            const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen,
+                            size_t n, uint8_t *d_work, size_t work_pitch);
+int dil_keypair_graph_create(dil_graph **graph, int mode, uint8_t *d_pk, size_t pk_pitch,
+                             uint8_t *d_sk, size_t sk_pitch, const uint8_t *d_seeds,
+                             size_t seed_pitch, uint64_t rand_index, size_t n, uint8_t *d_work,
+                             size_t work_pitch);
+int dil_graph_launch(dil_graph *graph, void *stream);
+size_t dil_graph_batch(const dil_graph *graph);
+int dil_graph_destroy(dil_graph *graph);
+
+/* ---------------------------------------------------------------------- */
+/* Serving loop (no counterpart in the reference, whose callers batch by   */
+/* hand, benches/bench_dilithium.cu:186-239).  Single requests from any    */
+/* host thread are batched continuously per (op, mode) onto `lanes`        */
+/* in-flight batches, each with device buffers preallocated from a fixed   */
+/* HBM budget; sign and verify use resident keys.                         */
+/* ---------------------------------------------------------------------- */
+
+#define DIL_OP_KEYGEN 0
+#define DIL_OP_SIGN 1
+#define DIL_OP_VERIFY 2
+
+typedef struct dil_server_config {
+  size_t hbm_budget_bytes;  /* device bytes for in-flight batches (all lanes) */
+  uint32_t lanes;           /* batches in flight at once (>= 1) */
+  uint32_t max_batch;       /* requests per batch (<= 65536) */
+  uint32_t max_keys[3];     /* resident key capacity of modes 2, 3, 5 */
+  uint32_t batch_window_us; /* hold a partial batch this long for more requests */
+} dil_server_config;
+
+/* One request.  The caller owns it and every buffer it points to until
+ * `done` reads 1; the server writes the output fields first.
+ *   keygen: seed (32 bytes) -> out = pk || sk
+ *   sign:   msg, mlen, resident key `key` -> out = signature
+ *   verify: msg, mlen, sig (signature bytes), resident key `key` -> verdict */
+typedef struct dil_request {
+  int32_t op;
+  int32_t mode;
+  uint32_t key;
+  uint32_t mlen;
+  const uint8_t *msg;
+  const uint8_t *seed;
+  const uint8_t *sig;
+  uint8_t *out;
+  int32_t status;   /* 0, or DIL_E* (DIL_ERANGE: larger than a lane admits) */
+  int32_t verdict;  /* verify: 0 valid, -1 invalid */
+  uint64_t t_submit_ns, t_launch_ns, t_done_ns; /* CLOCK_MONOTONIC */
+  uint32_t batch_id;
+  int32_t done;     /* set to 1 (release) when complete */
+} dil_request;
+
+typedef struct dil_server_stats {
+  uint64_t submitted, completed, batches, rejected;
+  uint64_t items_by_op[3];
+  uint64_t lane_bytes;      /* device bytes per lane */
+  uint64_t max_msg_bytes;   /* largest total message bytes one batch admits */
+} dil_server_stats;
+
+typedef struct dil_server dil_server;
+int dil_server_create(dil_server **srv, const dil_server_config *cfg);
+/* Completes every submitted request, then frees the server. */
+int dil_server_destroy(dil_server *srv);
+/* The resident key store of `mode` (load or generate keys into it before
+ * submitting requests that name them). */
+dil_keystore *dil_server_keystore(dil_server *srv, int mode);
+/* Queues n requests (stamps t_submit_ns). */
+int dil_server_submit(dil_server *srv, dil_request *const *reqs, size_t n);
+/* Waits until `req` is done (timeout_us < 0: no limit); 0 or DIL_ERANGE on
+ * timeout. */
+int dil_server_wait(dil_server *srv, dil_request *req, int64_t timeout_us);
+/* Waits until every request submitted so far is done. */
+int dil_server_drain(dil_server *srv);
+int dil_server_get_stats(dil_server *srv, dil_server_stats *stats);
+/* Load generator: submits reqs[i] at (start + arrival_ns[i]) from the
+ * calling thread, then returns (does not wait for completion). */
+int dil_server_replay(dil_server *srv, dil_request *reqs, const uint64_t *arrival_ns, size_t n);
+uint64_t dil_monotonic_ns(void);
+
 /* ---------------------------------------------------------------------- */
 /* Standalone NTT (include/ntt.cuh:30-62, src/ntt.cu:77-207)              */
 /* Polynomials are 256 int32 coefficients; consecutive polynomials start  */

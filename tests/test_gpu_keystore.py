@@ -1,0 +1,79 @@
+"""Resident key stores (dil_keystore_*): keygen into the store, loading host
+keys, and signing / verifying under resident keys give the same bytes and
+verdicts as the oracle (and hence as the non-resident entry points)."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _msgs(seed, n, lo, hi):
+    rng = np.random.default_rng(seed)
+    return [rng.integers(0, 256, int(rng.integers(lo, hi + 1)), dtype=np.uint8).tobytes() for _ in range(n)]
+
+
+@pytest.mark.parametrize("mode", [2, 3, 5])
+def test_keystore_keygen_sign_verify_match_oracle(mode):
+    from cudilithium_amd import api
+    nkeys, n = 24, 120
+    seeds = np.random.default_rng(500 + mode).integers(0, 256, (nkeys, 32), dtype=np.uint8)
+    epk, esk = oracle.keypair_batch(mode, seeds)
+    ks = api.KeyStore(mode, nkeys)
+    ks.keygen(0, seeds)
+    pk, sk = ks.export(0, nkeys)
+    assert np.array_equal(pk, epk) and np.array_equal(sk, esk)
+
+    rng = np.random.default_rng(mode)
+    key_idx = rng.integers(0, nkeys, n).astype(np.uint32)
+    msgs = _msgs(600 + mode, n, 0, 900)
+    exp, _ = oracle.sign_batch(mode, msgs, esk[key_idx])
+    for engine in (api.ENGINE_PERSISTENT, api.ENGINE_PERSTAGE):
+        ctx = api.SignContext(mode, n, 64 if engine == api.ENGINE_PERSTAGE else n, resident_only=True)
+        got = ctx.sign_resident(ks, key_idx, msgs, engine)
+        assert np.array_equal(got, exp), engine
+
+    sig = exp.copy()
+    flip = np.arange(3, n, 5)
+    for i in flip:
+        b = int(rng.integers(0, sig.shape[1] * 8))
+        sig[i, b // 8] ^= 1 << (b % 8)
+    ret = ks.verify(key_idx, sig, msgs)
+    oret = oracle.verify_batch(mode, sig, msgs, epk[key_idx])
+    assert np.array_equal(ret, oret)
+    assert (ret[flip] == -1).all() and (np.delete(ret, flip) == 0).all()
+
+
+def test_keystore_load_host_keys_and_verify_only_keys():
+    from cudilithium_amd import api
+    mode, nkeys, n = 3, 10, 40
+    pk, sk = oracle.keypair_counter_batch(mode, nkeys, rand_index=900)
+    full = api.KeyStore(mode, nkeys)
+    full.load(0, pk[:6], sk[:6])
+    full.load(6, pk[6:], sk[6:])
+    key_idx = (np.arange(n) * 7 % nkeys).astype(np.uint32)
+    msgs = _msgs(11, n, 1, 3000)
+    exp, _ = oracle.sign_batch(mode, msgs, sk[key_idx])
+    ctx = api.SignContext(mode, n, n)
+    assert np.array_equal(ctx.sign_resident(full, key_idx, msgs), exp)
+    # the non-resident path on the same context still works
+    assert np.array_equal(ctx.sign(msgs, sk[key_idx]), exp)
+
+    vonly = api.KeyStore(mode, nkeys)
+    vonly.load(0, pk)  # public keys only
+    ret = vonly.verify(key_idx, exp, msgs)
+    assert (ret == 0).all()
+    # wrong key for every item, and an index past the store's capacity
+    bad_idx = ((key_idx + 1) % nkeys).astype(np.uint32)
+    bad_idx[5] = nkeys + 3
+    assert (vonly.verify(bad_idx, exp, msgs) == -1).all()
+
+
+def test_resident_only_context_rejects_plain_sign():
+    from cudilithium_amd import api, _lib
+    mode = 2
+    ctx = api.SignContext(mode, 4, 4, resident_only=True)
+    _, sk = oracle.keypair_counter_batch(mode, 4)
+    with pytest.raises(_lib.DilError):
+        ctx.sign([b"x"] * 4, sk)
