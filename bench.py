@@ -10,7 +10,7 @@ dominant kernel (the persistent rejection loop, integer-VALU bound), the
 oracle's CPU rate on a bounded sample, and the per-stage (reference
 architecture) GPU form on the same inputs.
 
-    python bench.py [--steps K] [--warmup W] [--workload sign3|sign2|sign5|verify5|ntt|copy]
+    python bench.py [--steps K] [--warmup W] [--workload sign3|sign2|sign5|verify5|ntt|copy|serve]
 """
 from __future__ import annotations
 
@@ -79,17 +79,35 @@ def bench_sign(args, mode: int):
     wall = time.perf_counter() - t0
     tim = ctx.timings()
     ctx.enable_timing(0)
-    sig = d_sig.download((n, sigb), np.uint8)
-    attempts = ctx.attempt_counts(n)
+    sig_eager = d_sig.download((n, sigb), np.uint8)
     log(f"[bench] persistent: {args.steps} steps in {wall * 1e3:.2f} ms; loop kernel "
         f"{np.mean([t[1] for t in tim]):.3f} ms, fixed stage {np.mean([t[0] for t in tim]):.3f} ms")
+    eager_value = n * args.steps / wall
+
+    # the same call captured as a HIP graph (fixed stage's three branches +
+    # the persistent loop), replayed once per step: the headline figure
+    graph = ctx.graph(d_sig, d_sk, d_msgs)
+    d_sig.memset(0)
+    for _ in range(max(1, args.warmup)):
+        graph.launch()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch()
+    synchronize()
+    wall = time.perf_counter() - t0
+    sig = d_sig.download((n, sigb), np.uint8)
+    attempts = ctx.attempt_counts(n)
+    graph_matches_eager = bool(np.array_equal(sig, sig_eager))
+    log(f"[bench] graph replay: {args.steps} steps in {wall * 1e3:.2f} ms; bytes equal eager: {graph_matches_eager}")
 
     # parity on a sample (outside the timed region)
     import oracle
     rng = np.random.default_rng(1)
     idx = np.sort(rng.choice(n, size=min(n, args.check), replace=False))
     exp, exp_att = oracle.sign_batch(mode, [msgs[i] for i in idx], sk[idx])
-    parity = bool(np.array_equal(sig[idx], exp)) and bool(np.array_equal(attempts[idx], exp_att))
+    parity = bool(np.array_equal(sig[idx], exp)) and bool(np.array_equal(attempts[idx], exp_att)) \
+        and graph_matches_eager
     log(f"[bench] parity vs oracle on {len(idx)} items: {parity}")
 
     # roofline of the rejection loop: algorithmic int32 ops of the attempts the
@@ -121,7 +139,7 @@ def bench_sign(args, mode: int):
         cpu = cpu_baseline_sign(mode, msgs, sk, args.cpu_seconds)
 
     cfg = {"workload": f"dilithium{mode}_sign_{n}_msg_loguniform_32B-4KiB",
-           "batch": n, "mode": mode, "engine": "persistent", "slots": nslots,
+           "batch": n, "mode": mode, "engine": "persistent+graph", "slots": nslots,
            "slots_per_block": ctx.slots_per_block or "auto", "occupancy": ctx.occupancy or "auto",
            "mean_attempts": float(attempts.mean())}
     return {
@@ -131,7 +149,7 @@ def bench_sign(args, mode: int):
                      "frac": achieved / PEAK_INT32_TOPS, "traffic": TRAFFIC_BYTES.get(cfg["workload"]),
                      "kernel": "sg_persistent_kernel", "kernel_ms": loop_ms,
                      "algorithmic_ops": ops},
-        "cpu_baseline": cpu, "perstage_gpu_baseline": perstage,
+        "cpu_baseline": cpu, "perstage_gpu_baseline": perstage, "eager_persistent_value": eager_value,
         "speedup_vs_perstage": (value / perstage["value"]) if perstage else None,
     }
 
@@ -182,6 +200,17 @@ def bench_verify(args, mode: int = 5):
     for _ in range(args.steps):
         run()
     synchronize()
+    eager_wall = time.perf_counter() - t0
+    ret_eager = d_ret.download((n,), np.int32)
+    # the three launches captured as a HIP graph, replayed once per step
+    graph = api.verify_graph(mode, d_ret, d_sig, d_pk, dm, d_work, wrow)
+    d_ret.memset(0x55)
+    graph.launch()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch()
+    synchronize()
     wall = time.perf_counter() - t0
     ret = d_ret.download((n,), np.int32)
     exp = np.zeros(n, np.int32)
@@ -189,14 +218,16 @@ def bench_verify(args, mode: int = 5):
     import oracle
     sample = np.random.default_rng(2).choice(n, 256, replace=False)
     oret = oracle.verify_batch(mode, sig[sample], [mlist[i] for i in sample], pk[sample])
-    parity = bool((ret[sample] == oret).all()) and bool((ret[np.setdiff1d(np.arange(n), flip_idx)] == 0).all())
+    parity = bool((ret[sample] == oret).all()) and bool((ret[np.setdiff1d(np.arange(n), flip_idx)] == 0).all()) \
+        and bool((ret[flip_idx] == -1).all()) and bool(np.array_equal(ret, ret_eager))
     value = n * args.steps / wall
     ops = n * workmodel.verify_ops(mode, 1024)
     achieved = ops / (wall / args.steps) / 1e12
     return {"metric": METRIC, "value": value, "unit": "verifies/s", "ms_per_step": wall / args.steps * 1e3,
             "dtype": "int32", "parity": "bit-exact" if parity else "MISMATCH",
             "config": {"workload": f"dilithium{mode}_verify_{n}_msg_1KiB_10pct_bitflips", "batch": n, "mode": mode,
-                       "rejected": int((ret != 0).sum())},
+                       "engine": "graph", "rejected": int((ret != 0).sum())},
+            "eager_value": n * args.steps / eager_wall,
             "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
                          "frac": achieved / PEAK_INT32_TOPS, "traffic": None, "kernel": "verify pipeline (3 launches)"},
             "cpu_baseline": None}
@@ -230,6 +261,110 @@ def bench_ntt(args):
                          "unit": "TOP/s", "frac": ops / (ms * 1e-3) / 1e12 / PEAK_INT32_TOPS,
                          "traffic": None, "hbm_gbs": n * 3072 / (ms * 1e-3) / 1e9},
             "cpu_baseline": None}
+
+
+def _pct(v, q):
+    return float(np.percentile(v, q)) if len(v) else None
+
+
+def bench_serve(args):
+    """Serving mix (continuous batching, resident keys): saturation throughput
+    with every request queued at once, then p50 / p99 latency (submit ->
+    completion visible on the host) and throughput at 30 / 60 / 90 % of it
+    with Poisson arrivals.  Parity: a sample of one run's results against
+    the oracle, and a few requests re-run alone against their in-mix bytes."""
+    from cudilithium_amd import api, workloads
+    n_keys, n = 4096, args.requests
+    budget = int(args.budget_gib * (1 << 30))
+    keys = workloads.config5_keys(n_keys)
+    caps = tuple(len(keys[m][0]) for m in workloads.SERVE_MODES)
+    srv = api.Server(budget, lanes=args.lanes, max_batch=args.max_batch, max_keys=caps,
+                     batch_window_us=args.window_us)
+    for m in workloads.SERVE_MODES:
+        srv.stores[m].keygen(0, keys[m][1])
+    st0 = srv.stats()
+    log(f"[serve] {n_keys} resident keys, budget {budget >> 20} MiB over {args.lanes} lanes "
+        f"(max message bytes per batch {st0.max_msg_bytes >> 20} MiB), {n} requests per run")
+
+    def make(rate, off):
+        c = workloads.config5(n, rate, n_keys, seed_offset=off)
+        sigs = [None] * n
+        for m in workloads.SERVE_MODES:
+            sel = np.nonzero((c["ops"] == api.OP_VERIFY) & (c["modes"] == m))[0]
+            if len(sel):
+                ctx = api.SignContext(m, len(sel), resident_only=True)
+                sg = ctx.sign_resident(srv.stores[m], c["keys"][sel].astype(np.uint32), [c["msgs"][i] for i in sel])
+                for j, i in enumerate(sel):
+                    sigs[i] = sg[j]
+        return c, api.RequestBatch(c["ops"], c["modes"], c["keys"], c["msgs"], sigs, c["seeds"])
+
+    def summarize(b, c):
+        sub, done = b.field("t_submit_ns").astype(np.int64), b.field("t_done_ns").astype(np.int64)
+        lat = (done - sub) / 1e3
+        status = b.field("status")
+        out = {"requests": n, "errors": int((status != 0).sum()),
+               "throughput_rps": n / ((done.max() - sub.min()) / 1e9),
+               "p50_us": _pct(lat, 50), "p99_us": _pct(lat, 99), "mean_us": float(lat.mean())}
+        for op, name in ((0, "keygen"), (1, "sign"), (2, "verify")):
+            sel = c["ops"] == op
+            out[f"p99_us_{name}"] = _pct(lat[sel], 99)
+        return out
+
+    c, b = make(1e9, 0)
+    srv.submit(b.reqs)
+    srv.drain()
+    sat = summarize(b, c)
+    log(f"[serve] saturation {sat['throughput_rps']:.0f} req/s")
+    runs = []
+    for k, frac in enumerate((0.3, 0.6, 0.9)):
+        c, b = make(frac * sat["throughput_rps"], k + 1)
+        srv.replay(b.reqs, c["arrival_ns"])
+        srv.drain()
+        r = summarize(b, c)
+        r["offered_rps"] = frac * sat["throughput_rps"]
+        r["offered_fraction"] = frac
+        runs.append(r)
+        log(f"[serve] offered {r['offered_rps']:.0f} req/s: p50 {r['p50_us']:.0f} us, p99 {r['p99_us']:.0f} us, "
+            f"{r['throughput_rps']:.0f} req/s")
+
+    # parity of the last run on a sample, and isolation (alone vs in the mix)
+    import oracle
+    rng = np.random.default_rng(5)
+    sample = rng.choice(n, size=min(n, args.check), replace=False)
+    ok = True
+    for m in workloads.SERVE_MODES:
+        pk, sk = srv.stores[m].export(0, len(keys[m][0]))
+        for op in (api.OP_KEYGEN, api.OP_SIGN, api.OP_VERIFY):
+            sel = [i for i in sample if c["ops"][i] == op and c["modes"][i] == m]
+            if not sel:
+                continue
+            if op == api.OP_KEYGEN:
+                epk, esk = oracle.keypair_batch(m, c["seeds"][sel])
+                ok &= all(np.array_equal(b.out[i], np.concatenate([epk[j], esk[j]])) for j, i in enumerate(sel))
+            elif op == api.OP_SIGN:
+                es, _ = oracle.sign_batch(m, [c["msgs"][i] for i in sel], sk[c["keys"][sel]])
+                ok &= all(np.array_equal(b.out[i], es[j]) for j, i in enumerate(sel))
+            else:
+                ev = oracle.verify_batch(m, np.stack([b._sigs_by_index[i] for i in sel]), [c["msgs"][i] for i in sel],
+                                         pk[c["keys"][sel]])
+                ok &= all(b.reqs[i].verdict == ev[j] for j, i in enumerate(sel))
+    alone_ok = True
+    for i in sample[:24]:
+        one = api.RequestBatch(c["ops"][i:i + 1], c["modes"][i:i + 1], c["keys"][i:i + 1], c["msgs"][i:i + 1],
+                               [b._sigs_by_index.get(i)], c["seeds"][i:i + 1])
+        srv.submit(one.reqs)
+        srv.wait(one.reqs[0])
+        alone_ok &= (one.reqs[0].verdict == b.reqs[i].verdict) if c["ops"][i] == api.OP_VERIFY else \
+            bool(np.array_equal(one.out[0], b.out[i]))
+    st = srv.stats()
+    srv.close()
+    log(f"[serve] parity vs oracle on {len(sample)} requests: {ok}; alone == in mix on 24: {alone_ok}")
+    return {"metric": METRIC, "value": sat["throughput_rps"], "unit": "requests/s", "ms_per_step": None,
+            "dtype": "int32", "parity": "bit-exact" if (ok and alone_ok) else "MISMATCH",
+            "config": {"workload": f"serving_mix_{n_keys}_resident_keys_keygen20_sign50_verify30_msg_loguniform_32B-64KiB",
+                       "requests_per_run": n, "hbm_budget_bytes": budget, "lanes": args.lanes,
+                       "max_batch": args.max_batch, "batch_window_us": args.window_us, "batches": int(st.batches)},
+            "saturation": sat, "loads": runs, "roofline": None, "cpu_baseline": None}
 
 
 def bench_copy(args):
@@ -274,6 +409,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perstage", action="store_true")
+    ap.add_argument("--requests", type=int, default=40000, help="serve: requests per run")
+    ap.add_argument("--budget-gib", type=float, default=6.0, help="serve: HBM budget of the in-flight batches")
+    ap.add_argument("--lanes", type=int, default=3, help="serve: batches in flight")
+    ap.add_argument("--max-batch", type=int, default=4096, help="serve: requests per batch")
+    ap.add_argument("--window-us", type=int, default=0, help="serve: batching window")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -290,6 +430,8 @@ def main():
         res = bench_ntt(args)
     elif w == "copy":
         res = bench_copy(args)
+    elif w == "serve":
+        res = bench_serve(args)
     else:
         raise SystemExit(f"unknown workload {w}")
     res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,

@@ -89,6 +89,9 @@ def _register():
                                           c_void_p, c_size_t, c_int, c_void_p])
     R("dil_verify_device_resident", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p,
                                             c_void_p, c_size_t, c_void_p, c_size_t, c_void_p])
+    R("dil_sign_device_randomized", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t,
+                                            c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_int,
+                                            c_void_p])
     R("dil_graph_launch", c_int, [c_void_p, c_void_p])
     R("dil_graph_batch", c_size_t, [c_void_p])
     R("dil_graph_destroy", c_int, [c_void_p])
@@ -332,6 +335,24 @@ class SignContext:
         d_idx = DeviceBuffer.from_array(np.ascontiguousarray(key_idx, dtype=np.uint32))
         d_sig = DeviceBuffer(n * self.sig_bytes)
         self.sign_resident_device(store, d_idx, d_sig, dm, engine)
+        if engine == ENGINE_PERSISTENT and self.status() != 0:
+            raise _lib.DilError("persistent engine stopped at its round limit")
+        return d_sig.download((n, self.sig_bytes), np.uint8)
+
+    def sign_randomized(self, msgs, rhoprime: np.ndarray, sk: np.ndarray | None = None,
+                        store: "KeyStore | None" = None, key_idx=None, engine: int = ENGINE_PERSISTENT) -> np.ndarray:
+        """Randomized signatures: rhoprime[i] (64 bytes) for item i, under host
+        secret keys `sk` or resident keys `key_idx` of `store`."""
+        n = len(msgs)
+        dm = DeviceMessages(msgs)
+        d_rnd = DeviceBuffer.from_array(np.ascontiguousarray(rhoprime, dtype=np.uint8).reshape(n, 64))
+        d_sk = DeviceBuffer.from_array(np.ascontiguousarray(sk, dtype=np.uint8)) if sk is not None else None
+        d_idx = DeviceBuffer.from_array(np.ascontiguousarray(key_idx, dtype=np.uint32)) if store else None
+        d_sig = DeviceBuffer(n * self.sig_bytes)
+        _lib.check(L().dil_sign_device_randomized(self.handle, store.handle if store else None,
+                                                  d_idx.ptr if d_idx else None, d_sig.ptr, self.sig_bytes,
+                                                  d_sk.ptr if d_sk else None, self.sk_bytes, dm.data.ptr, dm.offs.ptr,
+                                                  dm.lens.ptr, d_rnd.ptr, 64, n, engine, None), "sign_randomized")
         if engine == ENGINE_PERSISTENT and self.status() != 0:
             raise _lib.DilError("persistent engine stopped at its round limit")
         return d_sig.download((n, self.sig_bytes), np.uint8)
@@ -586,6 +607,7 @@ class RequestBatch:
         out_bytes = [0] * n
         self.out = []
         self._sigs, self._seeds = [], []
+        self._sigs_by_index = {}
         for i in range(n):
             op, mode = int(ops[i]), int(modes[i])
             pkb, skb, sigb = sizes(mode)
@@ -603,6 +625,7 @@ class RequestBatch:
             else:
                 sg = np.ascontiguousarray(sigs[i], dtype=np.uint8)
                 self._sigs.append(sg)
+                self._sigs_by_index[i] = sg
                 r.sig = sg.ctypes.data
                 o = np.zeros(1, np.uint8)
             self.out.append(o)

@@ -318,7 +318,8 @@ struct dil_keystore {
 static int sign_enqueue(dil_sign_ctx *c, uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_sk, size_t sk_pitch,
                         const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, int engine,
                         hipStream_t st, bool timing, const dil_keystore *ks = nullptr,
-                        const uint32_t *d_key_idx = nullptr) {
+                        const uint32_t *d_key_idx = nullptr, const uint8_t *d_rnd = nullptr,
+                        size_t rnd_pitch = 0) {
     if (!c) return DIL_EINVAL;
     if (n == 0) return 0;
     if (n > c->max_batch) return DIL_ERANGE;
@@ -345,6 +346,8 @@ static int sign_enqueue(dil_sign_ctx *c, uint8_t *d_sig, size_t sig_pitch, const
         a.row_pitch = c->row_pitch;
         a.off_ints = SignLayout<P>::ints(0);
         a.n = n;
+        a.rnd = d_rnd;
+        a.rnd_pitch = rnd_pitch;
         if (ks) {
             a.key_idx = d_key_idx;
             a.key_ints = ks->rows + KeyLayout<P>::MAT;
@@ -724,3 +727,16 @@ int dil_verify_device_resident(const dil_keystore *k, const uint32_t *d_key_idx,
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------ randomized sign */
+
+extern "C" int dil_sign_device_randomized(dil_sign_ctx *c, const dil_keystore *ks, const uint32_t *d_key_idx,
+                                          uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_sk, size_t sk_pitch,
+                                          const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen,
+                                          const uint8_t *d_rhoprime, size_t rhoprime_pitch, size_t n, int engine,
+                                          void *stream) {
+    if (!d_rhoprime || rhoprime_pitch < 64) return DIL_EINVAL;
+    if (ks && !d_key_idx) return DIL_EINVAL;
+    return sign_enqueue(c, d_sig, sig_pitch, d_sk, sk_pitch, d_msg, d_moff, d_mlen, n, engine, S(stream), true, ks,
+                        d_key_idx, d_rhoprime, rhoprime_pitch);
+}

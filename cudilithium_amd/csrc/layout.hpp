@@ -33,7 +33,7 @@ template <class P>
 struct SlotLayout {
     using Z = Sizes<P>;
     static constexpr size_t W1PK = 0;                                 // K * POLYW1_BYTES
-    static constexpr size_t CP = align256(Z::W1PK_BYTES);             // int8[256]
+    static constexpr size_t CP = align256(Z::W1PK_BYTES);             // challenge masks (64 B)
     static constexpr size_t FLAG = CP + 256;                          // int32 flags
     static constexpr size_t YRAW = FLAG + 256;                        // L * 680 bytes
     static constexpr size_t W = YRAW + align256((size_t)P::L * Z::YRAW_BYTES);  // int32 K x 256
@@ -84,7 +84,7 @@ struct VerifyLayout {
     static constexpr size_t SIG = 0;
     static constexpr size_t PK = align256(Z::SIG_BYTES);
     static constexpr size_t HASH = PK + align256(Z::PK_BYTES);  // mu | w1' packed
-    static constexpr size_t CP = HASH + align256(Z::CHASH_IN_BYTES);  // int8[256]
+    static constexpr size_t CP = HASH + align256(Z::CHASH_IN_BYTES);  // challenge masks (64 B)
     static constexpr size_t HINT = CP + 256;                          // K x 256 bits
     static constexpr size_t FLAG = HINT + 256;                        // int32 status
     static constexpr size_t M = FLAG + 256;

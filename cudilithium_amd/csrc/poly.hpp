@@ -87,6 +87,18 @@ __device__ __forceinline__ void wave_load_D(int32_t (&v)[4], const int32_t *p) {
 __device__ __forceinline__ void wave_store_D(int32_t *p, const int32_t (&v)[4]) {
     reinterpret_cast<int4 *>(p)[lane_id()] = make_int4(v[0], v[1], v[2], v[3]);
 }
+// Challenge polynomial from its masks (nz[8] | neg[8], see challenge()) in
+// layout A: lane l, register r holds c[64 r + l] in {-1, 0, 1}.
+__device__ __forceinline__ void wave_load_challenge_A(int32_t (&v)[4], const uint32_t *masks) {
+    const int l = lane_id(), sh = l & 31, hi = l >> 5;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t nzw = masks[2 * r + hi], ngw = masks[8 + 2 * r + hi];
+        const int32_t nzb = (int32_t)((nzw >> sh) & 1u), ngb = (int32_t)((ngw >> sh) & 1u);
+        v[r] = nzb - 2 * (nzb & ngb);
+    }
+}
+
 __device__ __forceinline__ void wave_load_i8_A(int32_t (&v)[4], const int8_t *p) {
     const int l = lane_id();
 #pragma unroll

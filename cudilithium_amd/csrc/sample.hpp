@@ -120,21 +120,49 @@ __device__ __forceinline__ void mask_raw(const uint64_t (&rp)[8], uint32_t nonce
     }
 }
 
-// SampleInBall into a dense int8 polynomial in memory owned by this lane.
-template <int TAU>
-__device__ __forceinline__ void challenge(const uint64_t (&ct)[4], int8_t *cp) {
-    int4 *c4 = reinterpret_cast<int4 *>(cp);
+// SampleInBall with the polynomial held in registers as two 256-bit masks:
+// bit j of nz is set when c[j] != 0 and bit j of neg when c[j] = -1.  Each
+// step (c[i] = c[b]; c[b] = +-1) becomes mask-bit moves selected without
+// memory, so the byte loop has no dependent loads (the dense in-memory form
+// pays a memory round trip per step).  i >= N - TAU >= 192 lies in words
+// 6..7.  The masks are stored as nz[0..7] | neg[0..7] (64 bytes).
+__device__ __forceinline__ uint32_t mask_word(const uint32_t (&a)[8], uint32_t w) {
+    uint32_t r = a[0];
 #pragma unroll
-    for (int i = 0; i < N / 16; ++i) c4[i] = make_int4(0, 0, 0, 0);
+    for (int k = 1; k < 8; ++k) r = w == (uint32_t)k ? a[k] : r;
+    return r;
+}
+
+template <int TAU>
+__device__ __forceinline__ void challenge(const uint64_t (&ct)[4], uint32_t *masks) {
+    static_assert(N - TAU >= 192, "challenge positions i lie in mask words 6..7");
+    uint32_t nz[8], neg[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) nz[k] = neg[k] = 0;
     Lane64 s[25];
     shake_absorb_regs<SHAKE256_RATE, SEEDBYTES>(s, ct);
     keccak_f1600(s);
     uint64_t signs = from_lane(s[0]);
-    int i = N - TAU;
+    uint32_t i = N - TAU;
     auto step = [&](uint32_t b) {
-        if (i < N && b <= (uint32_t)i) {
-            cp[i] = cp[b];
-            cp[b] = (signs & 1) ? (int8_t)-1 : (int8_t)1;
+        if (i < (uint32_t)N && b <= i) {
+            const uint32_t wb = b >> 5, mb = 1u << (b & 31), mi = 1u << (i & 31);
+            const bool hi7 = (i >> 5) == 7;
+            // c[i] = c[b]
+            const uint32_t nzb = (mask_word(nz, wb) & mb) ? mi : 0u;
+            const uint32_t ngb = (mask_word(neg, wb) & mb) ? mi : 0u;
+            nz[6] = hi7 ? nz[6] : ((nz[6] & ~mi) | nzb);
+            nz[7] = hi7 ? ((nz[7] & ~mi) | nzb) : nz[7];
+            neg[6] = hi7 ? neg[6] : ((neg[6] & ~mi) | ngb);
+            neg[7] = hi7 ? ((neg[7] & ~mi) | ngb) : neg[7];
+            // c[b] = (-1)^(sign bit)
+            const uint32_t sb = (uint32_t)signs & 1u ? mb : 0u;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const bool here = wb == (uint32_t)k;
+                nz[k] = here ? (nz[k] | mb) : nz[k];
+                neg[k] = here ? ((neg[k] & ~mb) | sb) : neg[k];
+            }
             signs >>= 1;
             ++i;
         }
@@ -147,10 +175,15 @@ __device__ __forceinline__ void challenge(const uint64_t (&ct)[4], int8_t *cp) {
         for (int b = 0; b < 4; ++b, w >>= 8) step(w & 0xFFu);
     };
     static_for<2, SHAKE256_RATE / 4>(bytes_of);
-    while (i < N) {
+    while (i < (uint32_t)N) {
         keccak_f1600(s);
         static_for<0, SHAKE256_RATE / 4>(bytes_of);
     }
+    uint4 *o = reinterpret_cast<uint4 *>(masks);
+    o[0] = make_uint4(nz[0], nz[1], nz[2], nz[3]);
+    o[1] = make_uint4(nz[4], nz[5], nz[6], nz[7]);
+    o[2] = make_uint4(neg[0], neg[1], neg[2], neg[3]);
+    o[3] = make_uint4(neg[4], neg[5], neg[6], neg[7]);
 }
 
 // Squeezes the first output block after absorption and returns words [0, NW).

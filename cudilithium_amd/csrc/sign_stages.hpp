@@ -42,6 +42,10 @@ struct SignArgs {
     const uint8_t *key_ints;
     size_t key_pitch;
     uint32_t nkeys;
+    // Randomized signing (optional): item i's rhoprime is the 64 bytes at
+    // rnd + i * rnd_pitch instead of SHAKE256(key || mu) (api.cu:143-148).
+    const uint8_t *rnd;
+    size_t rnd_pitch;
 };
 
 __device__ __forceinline__ size_t sk_index(const SignArgs &a, size_t i) {
@@ -103,13 +107,19 @@ __device__ __forceinline__ void stage_mu(const SignArgs &a, size_t i) {
     uint64_t in[12];
     uint64_t mu[8];
     squeeze_words<8>(s, mu);
-#pragma unroll
-    for (int w = 0; w < 4; ++w) in[w] = key[w];
-#pragma unroll
-    for (int w = 0; w < 8; ++w) in[4 + w] = mu[w];
-    shake_absorb_regs<SHAKE256_RATE, SEEDBYTES + CRHBYTES>(s, in);
     uint64_t rp[8];
-    squeeze_words<8>(s, rp);
+    if (a.rnd) {
+        const uint8_t *r = a.rnd + i * a.rnd_pitch;
+#pragma unroll
+        for (int w = 0; w < 8; ++w) rp[w] = load_le64(r + 8 * w);
+    } else {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) in[w] = key[w];
+#pragma unroll
+        for (int w = 0; w < 8; ++w) in[4 + w] = mu[w];
+        shake_absorb_regs<SHAKE256_RATE, SEEDBYTES + CRHBYTES>(s, in);
+        squeeze_words<8>(s, rp);
+    }
     uint64_t *dst = reinterpret_cast<uint64_t *>(a.rows + i * a.row_pitch + SLy::MU);
 #pragma unroll
     for (int w = 0; w < 8; ++w) {
@@ -223,11 +233,16 @@ __device__ __forceinline__ void stage_c(const SignArgs &a, size_t item, uint8_t 
     for (int w = 0; w < 4; ++w)
 #pragma unroll
         for (int b = 0; b < 8; ++b) sig_out[8 * w + b] = (uint8_t)(ct[w] >> (8 * b));
-    challenge<P::TAU>(ct, reinterpret_cast<int8_t *>(slot + SL::CP));
+    challenge<P::TAU>(ct, reinterpret_cast<uint32_t *>(slot + SL::CP));
 }
 
 // Returns true (wave-uniform) when the attempt passes every check; then
-// sig_out holds the complete signature.
+// sig_out holds the complete signature.  An attempt is accepted iff all of
+// ||r0|| < gamma2 - beta, ||c t0|| < gamma2, #hints <= omega and
+// ||z|| < gamma1 - beta hold (sign.cu:986-1196), so the order of the checks
+// does not change any result; the r0 check runs first because it rejects
+// most often (about 2/3 of Dilithium3 attempts), then c t0 and the hints,
+// then z.  Fields of a rejected attempt's sig_out are left partly written.
 template <class P>
 __device__ __forceinline__ bool stage_z(const SignArgs &a, size_t item, uint8_t *slot, uint8_t *sig_out, int32_t *x,
                                         const ZetaLds &zt) {
@@ -236,26 +251,8 @@ __device__ __forceinline__ bool stage_z(const SignArgs &a, size_t item, uint8_t 
     const ItemView<P> it(a, item);
     const int lane = lane_id();
     int32_t ch[4];
-    wave_load_i8_A(ch, reinterpret_cast<const int8_t *>(slot + SL::CP));
+    wave_load_challenge_A(ch, reinterpret_cast<const uint32_t *>(slot + SL::CP));
     ntt_wave(ch, x, zt);
-
-    int32_t z[P::L][4];
-#pragma unroll
-    for (int l = 0; l < P::L; ++l) {
-        int32_t s[4], y[4];
-        wave_load_D(s, it.s1h + l * N);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) z[l][r] = mont_mul(ch[r], s[r]);
-        intt_wave(z[l], x, zt);
-        load_y_A<P>(y, slot, l);
-        bool big = false;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            z[l][r] = reduce32(z[l][r] + y[r]);
-            big |= exceeds(z[l][r], P::GAMMA1 - P::BETA);
-        }
-        if (wave_any(big)) return false;
-    }
 
     uint8_t *h = sig_out + Z::SIG_H;
     int pos = 0;
@@ -263,13 +260,13 @@ __device__ __forceinline__ bool stage_z(const SignArgs &a, size_t item, uint8_t 
         int32_t r0[4], ct0[4], w[4];
         wave_load_D(r0, it.s2h + k * N);
         wave_load_D(ct0, it.t0h + k * N);
+        wave_load_A(w, reinterpret_cast<const int32_t *>(slot + SL::W) + k * N);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             r0[r] = mont_mul(ch[r], r0[r]);
             ct0[r] = mont_mul(ch[r], ct0[r]);
         }
         intt_wave(r0, x, zt);
-        wave_load_A(w, reinterpret_cast<const int32_t *>(slot + SL::W) + k * N);
         int32_t w1[4];
         bool big = false;
 #pragma unroll
@@ -302,16 +299,28 @@ __device__ __forceinline__ bool stage_z(const SignArgs &a, size_t item, uint8_t 
         }
         if (lane == 0) h[P::OMEGA + k] = (uint8_t)pos;
     }
-    for (int j = pos + lane; j < P::OMEGA; j += 64) h[j] = 0;
 
-#pragma unroll
     for (int l = 0; l < P::L; ++l) {
-        layout_A_to_D(z[l], x);
+        int32_t zl[4], s[4], y[4];
+        wave_load_D(s, it.s1h + l * N);
+        load_y_A<P>(y, slot, l);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) zl[r] = mont_mul(ch[r], s[r]);
+        intt_wave(zl, x, zt);
+        bool big = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            zl[r] = reduce32(zl[r] + y[r]);
+            big |= exceeds(zl[r], P::GAMMA1 - P::BETA);
+        }
+        if (wave_any(big)) return false;
+        layout_A_to_D(zl, x);
         uint32_t f[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) f[r] = (uint32_t)(P::GAMMA1 - z[l][r]);
+        for (int r = 0; r < 4; ++r) f[r] = (uint32_t)(P::GAMMA1 - zl[r]);
         wave_pack_D<Z::Z_BITS>(sig_out + Z::SIG_Z + l * Z::POLYZ_BYTES, f);
     }
+    for (int j = pos + lane; j < P::OMEGA; j += 64) h[j] = 0;
     return true;
 }
 

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void vf_prep_kernel(VerifyArgs a) {
     uint64_t ct[4];
 #pragma unroll
     for (int w = 0; w < 4; ++w) ct[w] = load_le64(sig + 8 * w);
-    challenge<P::TAU>(ct, reinterpret_cast<int8_t *>(work + VL::CP));
+    challenge<P::TAU>(ct, reinterpret_cast<uint32_t *>(work + VL::CP));
 }
 
 template <class P>
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void vf_main_kernel(VerifyArgs a) {
             ntt_wave(v, sh.x[wave], sh.zt);
             wave_store_D(sh.zhat[g][l], v);
         } else {
-            wave_load_i8_A(v, reinterpret_cast<const int8_t *>(a.work + item * a.work_pitch + VL::CP));
+            wave_load_challenge_A(v, reinterpret_cast<const uint32_t *>(a.work + item * a.work_pitch + VL::CP));
             ntt_wave(v, sh.x[wave], sh.zt);
             wave_store_D(sh.chat[g], v);
         }

@@ -121,3 +121,49 @@ def config4(n: int = 65536, mlen: int = 1024, flip_fraction: float = 0.1):
     chosen = sorted(perm[:k])
     bits = [g.below(4595 * 8) for _ in chosen]
     return seeds, msgs, np.array(chosen, dtype=np.int64), np.array(bits, dtype=np.int64)
+
+
+SERVE_MODES = (2, 3, 5)
+
+
+def config5_keys(n_keys: int = 4096):
+    """Resident keypairs of the serving mix: key k has mode SERVE_MODES[k % 3]
+    and seed bytes [32 k, 32 k + 32) of the config's stream.  Returns, per
+    mode, (global key ids, seeds)."""
+    seeds = xoshiro_bytes(CONFIG_SEED[5], n_keys * 32).reshape(n_keys, 32)
+    out = {}
+    for j, m in enumerate(SERVE_MODES):
+        ids = np.arange(j, n_keys, len(SERVE_MODES))
+        out[m] = (ids, seeds[ids])
+    return out
+
+
+def config5(n_requests: int, rate_per_s: float, n_keys: int = 4096, seed_offset: int = 0,
+            lo: int = 32, hi: int = 65536):
+    """Serving mix: Poisson arrivals at rate_per_s (exponential gaps), ops
+    keygen / sign / verify with probability 0.2 / 0.5 / 0.3, mode uniform over
+    {2, 3, 5}, a resident key uniform among that mode's keys (index within
+    the mode's store), messages log-uniform in [lo, hi] bytes, 32-byte seeds
+    for keygen.  Returns dict of arrays / lists."""
+    g = Xoshiro256ss(CONFIG_SEED[5] + 100 + seed_offset)
+    per_mode = {m: len(range(j, n_keys, len(SERVE_MODES))) for j, m in enumerate(SERVE_MODES)}
+    ops = np.zeros(n_requests, np.int32)
+    modes = np.zeros(n_requests, np.int32)
+    keys = np.zeros(n_requests, np.int64)
+    lens = np.zeros(n_requests, np.int64)
+    gaps = np.zeros(n_requests, np.float64)
+    for i in range(n_requests):
+        u = g.below(10)
+        ops[i] = 0 if u < 2 else (1 if u < 7 else 2)
+        modes[i] = SERVE_MODES[g.below(3)]
+        keys[i] = g.below(per_mode[int(modes[i])])
+        lens[i] = g.log_uniform(lo, hi) if ops[i] != 0 else 0
+        gaps[i] = -math.log(1.0 - g.unit()) / rate_per_s
+    flat = xoshiro_bytes(CONFIG_SEED[5] + 200 + seed_offset, int(lens.sum()))
+    msgs, pos = [], 0
+    for ln in lens:
+        msgs.append(flat[pos:pos + int(ln)].tobytes())
+        pos += int(ln)
+    seeds = xoshiro_bytes(CONFIG_SEED[5] + 300 + seed_offset, n_requests * 32).reshape(n_requests, 32)
+    arrival_ns = (np.cumsum(gaps) * 1e9).astype(np.uint64)
+    return {"ops": ops, "modes": modes, "keys": keys, "msgs": msgs, "seeds": seeds, "arrival_ns": arrival_ns}

@@ -264,6 +264,19 @@ int dil_verify_device_resident(const dil_keystore *ks, const uint32_t *d_key_idx
                                const uint64_t *d_moff, const uint32_t *d_mlen, size_t n,
                                uint8_t *d_work, size_t work_pitch, void *stream);
 
+/* Randomized signing: the reference's DILITHIUM_RANDOMIZED_SIGNING build
+ * (api.cu:143-148), where rhoprime is 64 random bytes per item instead of
+ * SHAKE256(key || mu).  Here the caller supplies them (d_rhoprime, one 64-byte
+ * row per item) from the random source it trusts, so results are
+ * reproducible from the inputs.  With ks == NULL item i signs with the
+ * secret key at d_sk + i * sk_pitch (as dil_sign_device); with a key store,
+ * with resident key d_key_idx[i] (d_sk unused). */
+int dil_sign_device_randomized(dil_sign_ctx *ctx, const dil_keystore *ks, const uint32_t *d_key_idx,
+                               uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_sk, size_t sk_pitch,
+                               const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen,
+                               const uint8_t *d_rhoprime, size_t rhoprime_pitch, size_t n, int engine,
+                               void *stream);
+
 /* HIP graphs of whole pipelines (no counterpart in the reference, whose
  * host scheduler synchronises once per rejection round, api.cu:208-441).
  * Each *_graph_create captures every launch of one call of the matching

@@ -75,6 +75,8 @@ def lib():
                                                    ctypes.c_size_t, ctypes.c_int]
         L.oracle_sign_batch.argtypes = [ctypes.c_int, _u8p, _u8p, _u64p, _u64p, _u8p, ctypes.c_size_t,
                                         _i32p, ctypes.c_int]
+        L.oracle_sign_rnd_batch.argtypes = [ctypes.c_int, _u8p, _u8p, _u64p, _u64p, _u8p, _u8p,
+                                            ctypes.c_size_t, _i32p, ctypes.c_int]
         L.oracle_verify_batch.argtypes = [ctypes.c_int, _i32p, _u8p, _u8p, _u64p, _u64p, _u8p,
                                           ctypes.c_size_t, ctypes.c_int]
         L.oracle_max_threads.restype = ctypes.c_int
@@ -188,6 +190,19 @@ def sign_batch(mode: int, msgs, sk: np.ndarray, nthreads: int = 0):
     att = np.zeros(n, np.int32)
     lib().oracle_sign_batch(mode, _p(sig, _u8p), _p(flat, _u8p), _p(off, _u64p), _p(ln, _u64p),
                             _p(sk, _u8p), n, _p(att, _i32p), nthreads)
+    return sig, att
+
+
+def sign_rnd_batch(mode: int, msgs, sk: np.ndarray, rhoprime: np.ndarray, nthreads: int = 0):
+    """Randomized signatures with caller-chosen 64-byte rhoprime per item."""
+    p = params(mode)
+    flat, off, ln, n = _msgs(msgs)
+    sk = np.ascontiguousarray(sk, dtype=np.uint8)
+    rnd = np.ascontiguousarray(rhoprime, dtype=np.uint8).reshape(n, 64)
+    sig = np.zeros((n, p.sig_bytes), np.uint8)
+    att = np.zeros(n, np.int32)
+    lib().oracle_sign_rnd_batch(mode, _p(sig, _u8p), _p(flat, _u8p), _p(off, _u64p), _p(ln, _u64p),
+                                _p(sk, _u8p), _p(rnd, _u8p), n, _p(att, _i32p), nthreads)
     return sig, att
 
 

@@ -560,6 +560,14 @@ void oracle_keypair_counter(int mode, uint8_t *pk, uint8_t *sk, uint64_t ctr) {
 /* kernels of sign.cu.  Result = attempt with the first successful nonce.   */
 /* ------------------------------------------------------------------------ */
 int oracle_sign(int mode, uint8_t *sig, const uint8_t *m, size_t mlen, const uint8_t *sk) {
+    return oracle_sign_rnd(mode, sig, m, mlen, sk, NULL);
+}
+
+/* rnd != NULL: randomized signing, rhoprime = rnd (64 bytes), as the
+ * reference does under DILITHIUM_RANDOMIZED_SIGNING (api.cu:143-148, there
+ * from randombytes); NULL: deterministic rhoprime = SHAKE256(key || mu). */
+int oracle_sign_rnd(int mode, uint8_t *sig, const uint8_t *m, size_t mlen, const uint8_t *sk,
+                    const uint8_t *rnd) {
     oracle_params_t p;
     oracle_params(mode, &p);
     const uint8_t *rho = sk, *key = sk + SEEDBYTES, *tr = sk + 2 * SEEDBYTES;
@@ -594,7 +602,10 @@ int oracle_sign(int mode, uint8_t *sig, const uint8_t *m, size_t mlen, const uin
         sponge_squeeze(&st, mu, CRHBYTES);
     }
     memcpy(keymu, key, SEEDBYTES);
-    oracle_shake256(rhoprime, CRHBYTES, keymu, sizeof(keymu));
+    if (rnd)
+        memcpy(rhoprime, rnd, CRHBYTES);
+    else
+        oracle_shake256(rhoprime, CRHBYTES, keymu, sizeof(keymu));
 
     const unsigned w1bits = p.gamma2 == (Q - 1) / 88 ? 6 : 4;
     const unsigned zbits = p.gamma1 == (1 << 17) ? 18 : 20;
@@ -796,6 +807,22 @@ void oracle_sign_batch(int mode, uint8_t *sig, const uint8_t *msgs, const uint64
     for (long long i = 0; i < (long long)n; ++i) {
         int a = oracle_sign(mode, sig + (size_t)i * p.sig_bytes, msgs + moff[i], (size_t)mlen[i],
                             sk + (size_t)i * p.sk_bytes);
+        if (attempts) attempts[i] = a;
+    }
+}
+
+void oracle_sign_rnd_batch(int mode, uint8_t *sig, const uint8_t *msgs, const uint64_t *moff,
+                           const uint64_t *mlen, const uint8_t *sk, const uint8_t *rnd, size_t n,
+                           int32_t *attempts, int nthreads) {
+    oracle_params_t p;
+    oracle_params(mode, &p);
+    keccak_init_tables();
+    zetas_init();
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic)
+    for (long long i = 0; i < (long long)n; ++i) {
+        int a = oracle_sign_rnd(mode, sig + (size_t)i * p.sig_bytes, msgs + moff[i], (size_t)mlen[i],
+                                sk + (size_t)i * p.sk_bytes, rnd + (size_t)i * CRHBYTES);
         if (attempts) attempts[i] = a;
     }
 }

@@ -75,6 +75,10 @@ void oracle_keypair_counter(int mode, uint8_t *pk, uint8_t *sk, uint64_t ctr);
 /* Deterministic signature (api.cu:63-452 + sign.cu).  Returns the number
  * of attempts (the first successful nonce + 1). */
 int oracle_sign(int mode, uint8_t *sig, const uint8_t *m, size_t mlen, const uint8_t *sk);
+/* Randomized signature with the given 64-byte rhoprime (api.cu:143-148);
+ * rnd == NULL gives oracle_sign's deterministic signature. */
+int oracle_sign_rnd(int mode, uint8_t *sig, const uint8_t *m, size_t mlen, const uint8_t *sk,
+                    const uint8_t *rnd);
 /* 0 if valid, -1 otherwise (api.cu:454-515, verify.cu:28-325). */
 int oracle_verify(int mode, const uint8_t *sig, size_t siglen, const uint8_t *m, size_t mlen,
                   const uint8_t *pk);
@@ -89,6 +93,9 @@ void oracle_keypair_counter_batch(int mode, uint8_t *pk, uint8_t *sk, uint64_t r
 void oracle_sign_batch(int mode, uint8_t *sig, const uint8_t *msgs, const uint64_t *moff,
                        const uint64_t *mlen, const uint8_t *sk, size_t n, int32_t *attempts,
                        int nthreads);
+void oracle_sign_rnd_batch(int mode, uint8_t *sig, const uint8_t *msgs, const uint64_t *moff,
+                           const uint64_t *mlen, const uint8_t *sk, const uint8_t *rnd, size_t n,
+                           int32_t *attempts, int nthreads);
 void oracle_verify_batch(int mode, int32_t *ret, const uint8_t *sig, const uint8_t *msgs,
                          const uint64_t *moff, const uint64_t *mlen, const uint8_t *pk, size_t n,
                          int nthreads);

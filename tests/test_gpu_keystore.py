@@ -77,3 +77,23 @@ def test_resident_only_context_rejects_plain_sign():
     _, sk = oracle.keypair_counter_batch(mode, 4)
     with pytest.raises(_lib.DilError):
         ctx.sign([b"x"] * 4, sk)
+
+
+@pytest.mark.parametrize("mode", [2, 5])
+def test_randomized_signing_matches_oracle(mode):
+    """dil_sign_device_randomized with caller rhoprime: host keys and resident
+    keys, both engines, equal the oracle's randomized signatures."""
+    from cudilithium_amd import api
+    n, nkeys = 60, 6
+    pk, sk = oracle.keypair_counter_batch(mode, nkeys, rand_index=300)
+    key_idx = (np.arange(n) % nkeys).astype(np.uint32)
+    msgs = _msgs(900 + mode, n, 0, 2000)
+    rnd = np.random.default_rng(mode).integers(0, 256, (n, 64), dtype=np.uint8)
+    exp, _ = oracle.sign_rnd_batch(mode, msgs, sk[key_idx], rnd)
+    ctx = api.SignContext(mode, n, n)
+    assert np.array_equal(ctx.sign_randomized(msgs, rnd, sk=sk[key_idx]), exp)
+    assert np.array_equal(ctx.sign_randomized(msgs, rnd, sk=sk[key_idx], engine=api.ENGINE_PERSTAGE), exp)
+    ks = api.KeyStore(mode, nkeys)
+    ks.load(0, pk, sk)
+    assert np.array_equal(ctx.sign_randomized(msgs, rnd, store=ks, key_idx=key_idx), exp)
+    assert (oracle.verify_batch(mode, exp, msgs, pk[key_idx]) == 0).all()
