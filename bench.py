@@ -101,6 +101,16 @@ def bench_sign(args, mode: int):
     graph_matches_eager = bool(np.array_equal(sig, sig_eager))
     log(f"[bench] graph replay: {args.steps} steps in {wall * 1e3:.2f} ms; bytes equal eager: {graph_matches_eager}")
 
+    # phase profile of one more call, outside the timed region
+    ctx.profile_phases(True)
+    ctx.sign_device(d_sig, d_sk, d_msgs, api.ENGINE_PERSISTENT)
+    phases = ctx.phase_cycles()
+    ctx.profile_phases(False)
+    tot = sum(phases.values()) or 1
+    phase_share = {k: round(v / tot, 4) for k, v in phases.items()}
+    log(f"[bench] loop phase shares (workgroup cycles): {phase_share}; executed attempts "
+        f"{ctx.executed_attempts()}")
+
     # parity on a sample (outside the timed region)
     import oracle
     rng = np.random.default_rng(1)
@@ -141,7 +151,7 @@ def bench_sign(args, mode: int):
     cfg = {"workload": f"dilithium{mode}_sign_{n}_msg_loguniform_32B-4KiB",
            "batch": n, "mode": mode, "engine": "persistent+graph", "slots": nslots,
            "slots_per_block": ctx.slots_per_block or "auto", "occupancy": ctx.occupancy or "auto",
-           "mean_attempts": float(attempts.mean())}
+           "mean_attempts": float(attempts.mean()), "loop_phase_shares": phase_share}
     return {
         "metric": METRIC, "value": value, "unit": "signs/s", "ms_per_step": wall / args.steps * 1e3,
         "dtype": "int32", "config": cfg, "parity": "bit-exact" if parity else "MISMATCH",

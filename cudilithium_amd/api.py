@@ -92,6 +92,9 @@ def _register():
     R("dil_sign_device_randomized", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t,
                                             c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_int,
                                             c_void_p])
+    R("dil_sign_ctx_profile_phases", c_int, [c_void_p, c_int])
+    R("dil_sign_ctx_set_claim", c_int, [c_void_p, c_int])
+    R("dil_sign_ctx_phase_cycles", c_int, [c_void_p, c_void_p, c_void_p])
     R("dil_graph_launch", c_int, [c_void_p, c_void_p])
     R("dil_graph_batch", c_size_t, [c_void_p])
     R("dil_graph_destroy", c_int, [c_void_p])
@@ -300,6 +303,9 @@ class SignContext:
         _lib.check(L().dil_sign_ctx_set_tiling(self.handle, slots_per_block, occupancy, blocks), "set_tiling")
         self.slots_per_block, self.occupancy = slots_per_block, occupancy
 
+    def set_claim(self, items_per_block: int) -> None:
+        _lib.check(L().dil_sign_ctx_set_claim(self.handle, int(items_per_block)), "set_claim")
+
     def enable_timing(self, max_calls: int) -> None:
         _lib.check(L().dil_sign_ctx_enable_timing(self.handle, int(max_calls)), "enable_timing")
 
@@ -365,6 +371,16 @@ class SignContext:
                                              self.sk_bytes, msgs.data.ptr, msgs.offs.ptr, msgs.lens.ptr, msgs.n),
                    "sign_graph_create")
         return Graph(p.value, keep=(self, d_sig, d_sk, msgs))
+
+    PHASES = ("assign", "y", "w", "c", "z", "commit")
+
+    def profile_phases(self, enable: bool) -> None:
+        _lib.check(L().dil_sign_ctx_profile_phases(self.handle, 1 if enable else 0), "profile_phases")
+
+    def phase_cycles(self) -> dict:
+        out = np.zeros(6, np.uint64)
+        _lib.check(L().dil_sign_ctx_phase_cycles(self.handle, out.ctypes.data, None), "phase_cycles")
+        return dict(zip(self.PHASES, (int(v) for v in out)))
 
     def executed_attempts(self) -> int:
         v = ctypes.c_uint64(0)

@@ -20,7 +20,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
-SLOT_FACTORS = (0.5, 1.0, 2.0)
+SLOT_FACTORS = (0.5, 1.0, 1.5, 2.0)
 SLOTS_PER_BLOCK = (8, 16, 32)
 OCCUPANCY = (2, 3, 4)
 

@@ -153,7 +153,7 @@ static int sign_dropin(int mode, int engine, uint8_t *sig, size_t sig_pitch, siz
             if ((e = launch_sign_fixed(mode, a, st))) return e;
             uint32_t *ctl = reinterpret_cast<uint32_t *>(lut->d_copy_lut);
             e = launch_sign_persistent(mode, a, temp_slots, exec_threshold, default_slots_per_block(exec_threshold),
-                                       0, 0, ctl, st);
+                                       0, 0, 0, ctl, nullptr, st);
             if (e == -3) return DIL_ERANGE;
             if (e) return e;
             // like the reference's call, return once the rejection loop is done
@@ -305,6 +305,8 @@ struct dil_sign_ctx {
     std::vector<hipEvent_t> ev;
     size_t timing_cap = 0, timed = 0;
     bool resident_only = false;  // rows without the per-item expanded arrays
+    int claim = 0;               // items owned per workgroup (0 = slots per workgroup)
+    bool profile = false;        // phase sums into d_head words 16..27
 };
 
 // Resident keys: KeyLayout rows (pk, sk, tr, A, s1^, s2^, t0^, t1^).
@@ -369,7 +371,10 @@ static int sign_enqueue(dil_sign_ctx *c, uint8_t *d_sig, size_t sig_pitch, const
             if ((e = launch_sign_fixed(c->mode, a, st))) return e;
             if (e3 && (e = (int)hipEventRecord(e3[1], st))) return e;
             const int spb = c->slots_per_block ? c->slots_per_block : default_slots_per_block(c->nslots);
-            e = launch_sign_persistent(c->mode, a, slots, c->nslots, spb, c->occupancy, c->blocks, c->d_head, st);
+            unsigned long long *ph =
+                c->profile ? reinterpret_cast<unsigned long long *>(c->d_head + 16) : nullptr;
+            e = launch_sign_persistent(c->mode, a, slots, c->nslots, spb, c->occupancy, c->blocks, c->claim, c->d_head,
+                                       ph, st);
             if (e == -3) return DIL_ERANGE;
         }
         if (e) return e;
@@ -455,6 +460,12 @@ int dil_sign_ctx_set_tiling(dil_sign_ctx *c, int slots_per_block, int occupancy,
     return 0;
 }
 
+int dil_sign_ctx_set_claim(dil_sign_ctx *c, int items_per_block) {
+    if (!c || items_per_block < 0) return DIL_EINVAL;
+    c->claim = items_per_block;
+    return 0;
+}
+
 int dil_sign_ctx_enable_timing(dil_sign_ctx *c, int max_calls) {
     if (!c || max_calls < 0) return DIL_EINVAL;
     while (c->ev.size() < 3 * (size_t)max_calls) {
@@ -496,6 +507,18 @@ int dil_sign_ctx_executed_attempts(dil_sign_ctx *c, uint64_t *count, void *strea
     if (!e) e = (int)hipStreamSynchronize(S(stream));
     *count = v;
     return e;
+}
+
+int dil_sign_ctx_profile_phases(dil_sign_ctx *c, int enable) {
+    if (!c) return DIL_EINVAL;
+    c->profile = enable != 0;
+    return 0;
+}
+
+int dil_sign_ctx_phase_cycles(dil_sign_ctx *c, uint64_t *cycles, void *stream) {
+    if (!c || !cycles) return DIL_EINVAL;
+    int e = (int)hipMemcpyAsync(cycles, c->d_head + 16, 6 * sizeof(uint64_t), hipMemcpyDeviceToHost, S(stream));
+    return e ? e : (int)hipStreamSynchronize(S(stream));
 }
 
 int dil_sign_ctx_attempt_counts(dil_sign_ctx *c, uint32_t *h_counts, size_t n, void *stream) {

@@ -82,7 +82,8 @@ int sign_perstage(int mode, const SignArgs &a, const SlotRef &item_slots, const 
 // Rejection loop of the persistent engine over nslots slots (the fixed part
 // must have been launched first).  blocks <= 0: nslots / slots_per_block.
 int launch_sign_persistent(int mode, const SignArgs &a, const SlotRef &sr, size_t nslots, int slots_per_block,
-                           int occupancy, int blocks, uint32_t *d_head, hipStream_t st);
+                           int occupancy, int blocks, int claim, uint32_t *d_head, unsigned long long *phase_sums,
+                           hipStream_t st);
 int default_slots_per_block(size_t nslots);
 
 }  // namespace dil

@@ -6,12 +6,13 @@
 //   harvest  a lane whose completion event has fired scatters its outputs
 //            into the requests (signature bytes, verdict, pk || sk), stamps
 //            t_done and publishes done = 1
-//   launch   while a lane is free and requests wait, the group whose oldest
-//            request is oldest forms a batch: up to max_batch requests in
-//            FIFO order whose messages fit the lane's message area; inputs
-//            are staged into the lane's pinned buffer, copied in one
-//            transfer, the pipeline is enqueued on the lane's stream, the
-//            outputs copied back in one transfer and an event recorded
+//   launch   while a lane is free and requests wait, a batch takes the
+//            oldest requests across all (op, mode) groups, up to max_batch
+//            and the lane's message area; inputs are staged into the lane's
+//            pinned buffer and copied in one transfer, each group's pipeline
+//            runs on its own stream (forked from the lane's stream and joined
+//            back), the outputs come back in one transfer and an event marks
+//            completion
 // Each lane holds everything one batch needs (inputs, outputs, verify /
 // keygen work rows and one resident-only signing context per mode), carved
 // from the budget at creation, so admission never allocates.  Sign and
@@ -54,14 +55,17 @@ constexpr size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
 constexpr size_t MAX_SIG = Sizes<Params<5>>::SIG_BYTES;
 constexpr size_t MAX_KEYPAIR = Sizes<Params<5>>::PK_BYTES + Sizes<Params<5>>::SK_BYTES;
 
+// groups g = 3 * op + mode slot; a lane's batch holds any mix of them
 struct Lane {
-    hipStream_t st = nullptr;
-    hipEvent_t ev = nullptr;
+    hipStream_t st = nullptr;  // copies, fork and join
+    hipStream_t sub[9] = {};   // one stream per group's pipeline
+    hipEvent_t ev = nullptr, fork = nullptr, join[9] = {};
     uint8_t *h_in = nullptr, *h_out = nullptr;
     uint8_t *d_in = nullptr, *d_out = nullptr, *d_work = nullptr;
     dil_sign_ctx *ctx[3] = {nullptr, nullptr, nullptr};
-    std::vector<dil_request *> reqs;
-    int op = 0, mode = 0;
+    std::vector<dil_request *> reqs;  // ordered by group
+    size_t g0[10] = {};               // group g is reqs[g0[g], g0[g + 1])
+    size_t out_off[9] = {};           // group g's outputs in h_out / d_out
     bool busy = false;
     uint32_t batch_id = 0;
 };
@@ -124,6 +128,11 @@ void free_lane(Lane &ln) {
     (void)hipHostFree(ln.h_in);
     (void)hipHostFree(ln.h_out);
     if (ln.ev) (void)hipEventDestroy(ln.ev);
+    if (ln.fork) (void)hipEventDestroy(ln.fork);
+    for (int g = 0; g < 9; ++g) {
+        if (ln.join[g]) (void)hipEventDestroy(ln.join[g]);
+        if (ln.sub[g]) (void)hipStreamDestroy(ln.sub[g]);
+    }
     if (ln.st) (void)hipStreamDestroy(ln.st);
 }
 
@@ -136,50 +145,69 @@ size_t sign_ctx_bytes(int mode, size_t b) {
 
 }  // namespace
 
-// Takes the next batch for a free lane (called with mu held).  Returns false
-// when nothing is ready.
+// Takes the next batch for a free lane (called with mu held): requests of
+// every (op, mode) group, oldest first across groups, up to max_batch and the
+// lane's message area; then ordered by group so that each group is one
+// contiguous range.  Returns false when nothing is ready.
 bool dil_server::fill(Lane &ln) {
     const uint64_t now = mono_ns();
-    int bo = -1, bm = -1;
     uint64_t oldest = ~0ull;
+    size_t waiting = 0;
     for (int o = 0; o < 3; ++o)
         for (int m = 0; m < 3; ++m)
-            if (!q[o][m].empty() && q[o][m].front()->t_submit_ns < oldest) {
-                oldest = q[o][m].front()->t_submit_ns;
-                bo = o;
-                bm = m;
+            if (!q[o][m].empty()) {
+                oldest = std::min<uint64_t>(oldest, q[o][m].front()->t_submit_ns);
+                waiting += q[o][m].size();
             }
-    if (bo < 0) return false;
-    auto &dq = q[bo][bm];
+    if (waiting == 0) return false;
     // hold a partial batch for the window (it fills while other lanes run)
-    if (cfg.batch_window_us && dq.size() < cfg.max_batch && now - oldest < 1000ull * cfg.batch_window_us)
-        return false;
+    if (cfg.batch_window_us && waiting < cfg.max_batch && now - oldest < 1000ull * cfg.batch_window_us) return false;
     ln.reqs.clear();
-    ln.op = bo;
-    ln.mode = MODES[bm];
     size_t bytes = 0;
-    while (!dq.empty() && ln.reqs.size() < cfg.max_batch) {
-        dil_request *r = dq.front();
+    for (;;) {
+        int bo = -1, bm = -1;
+        uint64_t t = ~0ull;
+        for (int o = 0; o < 3; ++o)
+            for (int m = 0; m < 3; ++m)
+                if (!q[o][m].empty() && q[o][m].front()->t_submit_ns < t) {
+                    t = q[o][m].front()->t_submit_ns;
+                    bo = o;
+                    bm = m;
+                }
+        if (bo < 0 || ln.reqs.size() >= cfg.max_batch) break;
+        dil_request *r = q[bo][bm].front();
         const size_t need = bo == DIL_OP_KEYGEN ? 0 : al8(r->mlen);
         if (bytes + need > msg_cap) break;
         bytes += need;
         ln.reqs.push_back(r);
-        dq.pop_front();
+        q[bo][bm].pop_front();
     }
+    std::stable_sort(ln.reqs.begin(), ln.reqs.end(), [](const dil_request *a, const dil_request *b) {
+        return a->op * 3 + mode_slot(a->mode) < b->op * 3 + mode_slot(b->mode);
+    });
+    for (int g = 0; g <= 9; ++g) ln.g0[g] = 0;
+    for (const auto *r : ln.reqs) ++ln.g0[r->op * 3 + mode_slot(r->mode) + 1];
+    for (int g = 0; g < 9; ++g) ln.g0[g + 1] += ln.g0[g];
     npending -= ln.reqs.size();
     ln.batch_id = next_batch++;
     ++batches;
-    by_op[bo] += ln.reqs.size();
+    for (const auto *r : ln.reqs) ++by_op[r->op];
     return !ln.reqs.empty();
 }
 
-// Stages inputs and enqueues the pipeline of the lane's batch.
+namespace {
+size_t out_item_bytes(int op, int mode) {
+    if (op == DIL_OP_SIGN) return dil_signature_bytes(mode);
+    if (op == DIL_OP_VERIFY) return 4;
+    return dil_public_key_bytes(mode) + dil_secret_key_bytes(mode);
+}
+}  // namespace
+
+// Stages inputs and enqueues every group's pipeline of the lane's batch, each
+// on its own stream forked from and joined back into the lane's stream.
 int dil_server::enqueue(Lane &ln) {
     const size_t n = ln.reqs.size();
-    const int mode = ln.mode, ms = mode_slot(mode);
-    const size_t pkb = dil_public_key_bytes(mode), skb = dil_secret_key_bytes(mode), sigb = dil_signature_bytes(mode);
-    const size_t payload = ln.op == DIL_OP_VERIFY ? n * sigb : (ln.op == DIL_OP_KEYGEN ? n * 32 : 0);
-    const InLayout L(n, payload);
+    const InLayout L(n, n * MAX_SIG);
     uint32_t *idx = reinterpret_cast<uint32_t *>(ln.h_in + L.idx);
     uint64_t *moff = reinterpret_cast<uint64_t *>(ln.h_in + L.moff);
     uint32_t *mlen = reinterpret_cast<uint32_t *>(ln.h_in + L.mlen);
@@ -187,60 +215,80 @@ int dil_server::enqueue(Lane &ln) {
     size_t pos = 0;
     const uint64_t t = mono_ns();
     for (size_t i = 0; i < n; ++i) {
-        const dil_request *r = ln.reqs[i];
-        ln.reqs[i]->t_launch_ns = t;
-        ln.reqs[i]->batch_id = ln.batch_id;
+        dil_request *r = ln.reqs[i];
+        r->t_launch_ns = t;
+        r->batch_id = ln.batch_id;
         idx[i] = r->key;
         moff[i] = pos;
-        mlen[i] = ln.op == DIL_OP_KEYGEN ? 0 : r->mlen;
-        if (ln.op == DIL_OP_VERIFY) std::memcpy(pay + i * sigb, r->sig, sigb);
-        if (ln.op == DIL_OP_KEYGEN) std::memcpy(pay + i * 32, r->seed, 32);
-        if (ln.op != DIL_OP_KEYGEN && r->mlen) std::memcpy(ln.h_in + L.msgs + pos, r->msg, r->mlen);
-        if (ln.op != DIL_OP_KEYGEN) pos += al8(r->mlen);
+        mlen[i] = r->op == DIL_OP_KEYGEN ? 0 : r->mlen;
+        if (r->op == DIL_OP_VERIFY) std::memcpy(pay + i * MAX_SIG, r->sig, dil_signature_bytes(r->mode));
+        if (r->op == DIL_OP_KEYGEN) std::memcpy(pay + i * MAX_SIG, r->seed, 32);
+        if (r->op != DIL_OP_KEYGEN && r->mlen) std::memcpy(ln.h_in + L.msgs + pos, r->msg, r->mlen);
+        if (r->op != DIL_OP_KEYGEN) pos += al8(r->mlen);
     }
     int e = (int)hipMemcpyAsync(ln.d_in, ln.h_in, L.msgs + std::max<size_t>(pos, 8), hipMemcpyHostToDevice, ln.st);
     if (e) return e;
-    const uint32_t *d_idx = reinterpret_cast<const uint32_t *>(ln.d_in + L.idx);
-    const uint64_t *d_moff = reinterpret_cast<const uint64_t *>(ln.d_in + L.moff);
-    const uint32_t *d_mlen = reinterpret_cast<const uint32_t *>(ln.d_in + L.mlen);
+    if ((e = (int)hipEventRecord(ln.fork, ln.st))) return e;
     const uint8_t *d_msg = ln.d_in + L.msgs;
-    size_t out_bytes = 0;
-    if (ln.op == DIL_OP_SIGN) {
-        dil_sign_ctx *c = ln.ctx[ms];
-        // small batches: fewer slots per workgroup, so that more workgroups share them
-        int spb = n >= 8192 ? 32 : (n >= 2048 ? 16 : 8);
-        if ((size_t)spb > cfg.max_batch) spb = 1;
-        if ((e = dil_sign_ctx_set_tiling(c, spb, 0, 0))) return e;
-        e = dil_sign_device_resident(c, ks[ms], d_idx, ln.d_out, sigb, d_msg, d_moff, d_mlen, n,
-                                     DIL_ENGINE_PERSISTENT, ln.st);
-        out_bytes = n * sigb;
-    } else if (ln.op == DIL_OP_VERIFY) {
-        e = dil_verify_device_resident(ks[ms], d_idx, reinterpret_cast<int32_t *>(ln.d_out), ln.d_in + L.payload,
-                                       sigb, d_msg, d_moff, d_mlen, n, ln.d_work, work_pitch, ln.st);
-        out_bytes = n * 4;
-    } else {
-        e = dil_keypair_device(mode, ln.d_out, pkb + skb, ln.d_out + pkb, pkb + skb, ln.d_in + L.payload, 32, 0, n,
-                               ln.d_work, work_pitch, ln.st);
-        out_bytes = n * (pkb + skb);
+    size_t out = 0;
+    for (int g = 0; g < 9; ++g) {
+        const size_t g0 = ln.g0[g], ng = ln.g0[g + 1] - g0;
+        ln.out_off[g] = out;
+        if (ng == 0) continue;
+        const int op = g / 3, ms = g % 3, mode = MODES[ms];
+        const size_t pkb = dil_public_key_bytes(mode), skb = dil_secret_key_bytes(mode);
+        const size_t sigb = dil_signature_bytes(mode);
+        hipStream_t gs = ln.sub[g];
+        if ((e = (int)hipStreamWaitEvent(gs, ln.fork, 0))) return e;
+        const uint32_t *d_idx = reinterpret_cast<const uint32_t *>(ln.d_in + L.idx) + g0;
+        const uint64_t *d_moff = reinterpret_cast<const uint64_t *>(ln.d_in + L.moff) + g0;
+        const uint32_t *d_mlen = reinterpret_cast<const uint32_t *>(ln.d_in + L.mlen) + g0;
+        const uint8_t *d_pay = ln.d_in + L.payload + g0 * MAX_SIG;
+        uint8_t *d_out = ln.d_out + out;
+        uint8_t *d_work = ln.d_work + g0 * work_pitch;
+        if (op == DIL_OP_SIGN) {
+            dil_sign_ctx *c = ln.ctx[ms];
+            // small batches: fewer slots per workgroup, spread over all of them,
+            // so that each item gets spb / claim nonces per round
+            int spb = ng >= 8192 ? 32 : (ng >= 2048 ? 16 : 8);
+            if ((size_t)spb > cfg.max_batch) spb = 1;
+            if ((e = dil_sign_ctx_set_tiling(c, spb, 0, 0))) return e;
+            const size_t blocks = cfg.max_batch / (size_t)spb;
+            const int claim = (int)std::min<size_t>((size_t)spb, std::max<size_t>(1, (ng + blocks - 1) / blocks));
+            if ((e = dil_sign_ctx_set_claim(c, claim))) return e;
+            e = dil_sign_device_resident(c, ks[ms], d_idx, d_out, sigb, d_msg, d_moff, d_mlen, ng,
+                                         DIL_ENGINE_PERSISTENT, gs);
+        } else if (op == DIL_OP_VERIFY) {
+            e = dil_verify_device_resident(ks[ms], d_idx, reinterpret_cast<int32_t *>(d_out), d_pay, MAX_SIG, d_msg,
+                                           d_moff, d_mlen, ng, d_work, work_pitch, gs);
+        } else {
+            e = dil_keypair_device(mode, d_out, pkb + skb, d_out + pkb, pkb + skb, d_pay, MAX_SIG, 0, ng, d_work,
+                                   work_pitch, gs);
+        }
+        if (e) return e;
+        out += al8(ng * out_item_bytes(op, mode));
+        if ((e = (int)hipEventRecord(ln.join[g], gs))) return e;
+        if ((e = (int)hipStreamWaitEvent(ln.st, ln.join[g], 0))) return e;
     }
-    if (e) return e;
-    if ((e = (int)hipMemcpyAsync(ln.h_out, ln.d_out, out_bytes, hipMemcpyDeviceToHost, ln.st))) return e;
+    if ((e = (int)hipMemcpyAsync(ln.h_out, ln.d_out, out, hipMemcpyDeviceToHost, ln.st))) return e;
     return (int)hipEventRecord(ln.ev, ln.st);
 }
 
 void dil_server::finish(Lane &ln, int err) {
     const size_t n = ln.reqs.size();
-    const int mode = ln.mode;
-    const size_t pkb = dil_public_key_bytes(mode), skb = dil_secret_key_bytes(mode), sigb = dil_signature_bytes(mode);
-    for (size_t i = 0; i < n; ++i) {
-        dil_request *r = ln.reqs[i];
-        if (!err) {
-            if (ln.op == DIL_OP_SIGN) std::memcpy(r->out, ln.h_out + i * sigb, sigb);
-            if (ln.op == DIL_OP_KEYGEN) std::memcpy(r->out, ln.h_out + i * (pkb + skb), pkb + skb);
-            if (ln.op == DIL_OP_VERIFY) r->verdict = reinterpret_cast<const int32_t *>(ln.h_out)[i];
+    for (int g = 0; g < 9 && !err; ++g) {
+        const int op = g / 3, mode = MODES[g % 3];
+        const size_t sz = out_item_bytes(op, mode);
+        for (size_t i = ln.g0[g]; i < ln.g0[g + 1]; ++i) {
+            dil_request *r = ln.reqs[i];
+            const uint8_t *src = ln.h_out + ln.out_off[g] + (i - ln.g0[g]) * sz;
+            if (op == DIL_OP_VERIFY)
+                r->verdict = *reinterpret_cast<const int32_t *>(src);
+            else
+                std::memcpy(r->out, src, sz);
         }
-        publish(r, err);
     }
+    for (size_t i = 0; i < n; ++i) publish(ln.reqs[i], err);
     ln.reqs.clear();
     ln.busy = false;
     completed.fetch_add(n, std::memory_order_acq_rel);
@@ -300,7 +348,7 @@ int dil_server_create(dil_server **out, const dil_server_config *cfg) {
     (void)hipGetDevice(&s->device);
     const size_t b = cfg->max_batch;
     // per lane: outputs, fixed inputs, work rows, signing contexts, messages
-    s->out_cap = b * std::max(MAX_SIG, MAX_KEYPAIR);
+    s->out_cap = b * std::max(MAX_SIG, MAX_KEYPAIR) + 9 * 8;
     s->in_fixed = InLayout(b, b * MAX_SIG).msgs;
     size_t wrow = 0;
     for (int m : MODES)
@@ -324,6 +372,11 @@ int dil_server_create(dil_server **out, const dil_server_config *cfg) {
         };
         chk((int)hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking));
         chk((int)hipEventCreateWithFlags(&ln.ev, hipEventDisableTiming));
+        chk((int)hipEventCreateWithFlags(&ln.fork, hipEventDisableTiming));
+        for (int g = 0; g < 9; ++g) {
+            chk((int)hipStreamCreateWithFlags(&ln.sub[g], hipStreamNonBlocking));
+            chk((int)hipEventCreateWithFlags(&ln.join[g], hipEventDisableTiming));
+        }
         chk((int)hipHostMalloc((void **)&ln.h_in, s->in_fixed + s->msg_cap, hipHostMallocDefault));
         chk((int)hipHostMalloc((void **)&ln.h_out, s->out_cap, hipHostMallocDefault));
         chk((int)hipMalloc((void **)&ln.d_in, s->in_fixed + s->msg_cap));

@@ -3,21 +3,27 @@
 
 namespace dil {
 
-// control words: [0] queue head, [1] status, [2] attempts executed
-__global__ void zero_ctl_kernel(uint32_t *p) {
+// clears the queue head, status and attempt count, and the phase sums
+__global__ void zero_ctl_kernel(uint32_t *p, unsigned long long *phase_sums) {
     p[0] = 0;
     p[1] = 0;
     p[2] = 0;
+    if (phase_sums)
+        for (int w = 0; w < PERSIST_PHASES; ++w) phase_sums[w] = 0;
 }
 
 int launch_sign_persistent(int mode, const SignArgs &a, const SlotRef &sr, size_t nslots, int slots_per_block,
-                           int occupancy, int blocks, uint32_t *d_head, hipStream_t st) {
+                           int occupancy, int blocks, int claim, uint32_t *d_head, unsigned long long *phase_sums,
+                           hipStream_t st) {
     if (a.n == 0) return 0;
     if (occupancy == 0) occupancy = 2;
     switch (mode) {
-        case 2: return launch_persistent_p<Params<2>>(a, sr, nslots, slots_per_block, occupancy, blocks, d_head, st);
-        case 3: return launch_persistent_p<Params<3>>(a, sr, nslots, slots_per_block, occupancy, blocks, d_head, st);
-        case 5: return launch_persistent_p<Params<5>>(a, sr, nslots, slots_per_block, occupancy, blocks, d_head, st);
+        case 2: return launch_persistent_p<Params<2>>(a, sr, nslots, slots_per_block, occupancy, blocks, claim, d_head,
+                                                      phase_sums, st);
+        case 3: return launch_persistent_p<Params<3>>(a, sr, nslots, slots_per_block, occupancy, blocks, claim, d_head,
+                                                      phase_sums, st);
+        case 5: return launch_persistent_p<Params<5>>(a, sr, nslots, slots_per_block, occupancy, blocks, claim, d_head,
+                                                      phase_sums, st);
         default: return -1;
     }
 }

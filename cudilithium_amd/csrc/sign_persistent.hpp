@@ -32,11 +32,18 @@ namespace dil {
 
 constexpr int PERSIST_THREADS = 256;
 constexpr uint32_t PERSIST_MAX_ROUNDS = 20000;
+// control words (uint32): 0 queue head, 1 status, 2 attempts executed.
+// Optional phase profile: PERSIST_PHASES uint64 cycle sums (assign, y, w, c,
+// z, commit) at a separate pointer, null when not profiling.
+constexpr int PERSIST_PHASES = 6;
 
 // WPE: waves per SIMD the register allocation must allow (512 / WPE VGPRs)
+// claim: at most this many items owned per workgroup (1..S); below S every
+// owned item gets at least S / claim slots (nonces) per round, which trades
+// speculative attempts for fewer rounds on small batches
 template <class P, int S, int WPE>
 __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void sg_persistent_kernel(
-    SignArgs a, SlotRef sr, uint32_t *head) {
+    SignArgs a, SlotRef sr, uint32_t *head, int claim, unsigned long long *phase_sums) {
     using Z = Sizes<P>;
     __shared__ ZetaLds zt;
     __shared__ int32_t x[PERSIST_THREADS / 64][NTT_XBUF];
@@ -56,6 +63,17 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
         queue_empty = 0;
     }
     __syncthreads();
+    // optional phase profile: workgroup cycles per phase, summed over workgroups
+    const bool prof = phase_sums != nullptr;
+    uint64_t tph[PERSIST_PHASES] = {0, 0, 0, 0, 0, 0};
+    uint64_t tlast = prof ? clock64() : 0;
+    auto mark = [&](int ph) {
+        if (prof && tid == 0) {
+            const uint64_t t = clock64();
+            tph[ph] += t - tlast;
+            tlast = t;
+        }
+    };
 
     for (uint32_t round = 0;; ++round) {
         if (round == PERSIST_MAX_ROUNDS) {
@@ -65,15 +83,16 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
             break;
         }
         if (tid == 0) {
+            const int cap = claim > 0 && claim < S ? claim : S;
             int u = n_own;
-            if (u < S && !queue_empty) {
-                const uint32_t start = atomicAdd(head, (uint32_t)(S - u));
-                for (int j = 0; j < S - u && (size_t)start + j < a.n; ++j) {
+            if (u < cap && !queue_empty) {
+                const uint32_t start = atomicAdd(head, (uint32_t)(cap - u));
+                for (int j = 0; j < cap - u && (size_t)start + j < a.n; ++j) {
                     own_item[n_own] = start + j;
                     own_next[n_own] = 0;
                     ++n_own;
                 }
-                if ((size_t)start + (S - u) >= a.n) queue_empty = 1;
+                if ((size_t)start + (cap - u) >= a.n) queue_empty = 1;
             }
             u = n_own;
             for (int s = 0; s < S && u > 0; ++s) {
@@ -87,6 +106,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
             }
         }
         __syncthreads();
+        mark(0);
         const int u = n_own;
         if (u == 0) break;
         if (tid == 0) atomicAdd(head + 2, (uint32_t)S);  // attempts executed (incl. speculative)
@@ -96,15 +116,19 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
             stage_y<P>(a, own_item[slot_own[s]], slot_nonce[s], t % P::L, slot_base(s));
         }
         __syncthreads();
+        mark(1);
         for (int s = wave; s < S; s += NW) stage_w<P>(a, own_item[slot_own[s]], slot_base(s), x[wave], zt);
         __syncthreads();
+        mark(2);
         for (int s = tid; s < S; s += PERSIST_THREADS) stage_c<P>(a, own_item[slot_own[s]], slot_base(s), slot_sig(s));
         __syncthreads();
+        mark(3);
         for (int s = wave; s < S; s += NW) {
             const bool ok = stage_z<P>(a, own_item[slot_own[s]], slot_base(s), slot_sig(s), x[wave], zt);
             if (lane == 0) slot_ok[s] = ok ? 1 : 0;
         }
         __syncthreads();
+        mark(4);
         // commit: slots of item o are o, o + u, ... in increasing nonce order
         for (int o = wave; o < u; o += NW) {
             int win = -1;
@@ -137,21 +161,27 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
             n_own = k;
         }
         __syncthreads();
+        mark(5);
+    }
+    if (prof && tid == 0) {
+#pragma unroll
+        for (int p = 0; p < PERSIST_PHASES; ++p) atomicAdd(phase_sums + p, (unsigned long long)tph[p]);
     }
 }
 
-// queue head and status word; a kernel rather than a memset so that the
-// sequence can be captured in a HIP graph
-__global__ void zero_ctl_kernel(uint32_t *p);
+// clears the control words (and the phase sums when given); a kernel rather
+// than a memset so that the sequence can be captured in a HIP graph
+__global__ void zero_ctl_kernel(uint32_t *p, unsigned long long *phase_sums);
 
 template <class P, int S, int WPE>
-static int launch_persistent_s(const SignArgs &a, const SlotRef &sr, size_t nslots, int blocks, uint32_t *d_head,
-                               hipStream_t st) {
+static int launch_persistent_s(const SignArgs &a, const SlotRef &sr, size_t nslots, int blocks, int claim,
+                               uint32_t *d_head, unsigned long long *phase_sums, hipStream_t st) {
     int nb = (int)(nslots / S);
     if (blocks > 0 && blocks < nb) nb = blocks;
     if (nb < 1) return -3;
-    hipLaunchKernelGGL(zero_ctl_kernel, dim3(1), dim3(1), 0, st, d_head);
-    hipLaunchKernelGGL((sg_persistent_kernel<P, S, WPE>), dim3(nb), dim3(PERSIST_THREADS), 0, st, a, sr, d_head);
+    hipLaunchKernelGGL(zero_ctl_kernel, dim3(1), dim3(1), 0, st, d_head, phase_sums);
+    hipLaunchKernelGGL((sg_persistent_kernel<P, S, WPE>), dim3(nb), dim3(PERSIST_THREADS), 0, st, a, sr, d_head,
+                       claim, phase_sums);
     return (int)hipGetLastError();
 }
 
@@ -159,16 +189,16 @@ static int launch_persistent_s(const SignArgs &a, const SlotRef &sr, size_t nslo
 // SIMD the kernel's register allocation allows (2, 3 or 4).
 template <class P>
 int launch_persistent_p(const SignArgs &a, const SlotRef &sr, size_t nslots, int slots_per_block, int occupancy,
-                        int blocks, uint32_t *d_head, hipStream_t st);
+                        int blocks, int claim, uint32_t *d_head, unsigned long long *phase_sums, hipStream_t st);
 
 template <class P, int WPE>
 static int launch_persistent_w(const SignArgs &a, const SlotRef &sr, size_t nslots, int slots_per_block, int blocks,
-                               uint32_t *d_head, hipStream_t st) {
+                               int claim, uint32_t *d_head, unsigned long long *ph, hipStream_t st) {
     switch (slots_per_block) {
-        case 1: return launch_persistent_s<P, 1, WPE>(a, sr, nslots, blocks, d_head, st);
-        case 8: return launch_persistent_s<P, 8, WPE>(a, sr, nslots, blocks, d_head, st);
-        case 16: return launch_persistent_s<P, 16, WPE>(a, sr, nslots, blocks, d_head, st);
-        case 32: return launch_persistent_s<P, 32, WPE>(a, sr, nslots, blocks, d_head, st);
+        case 1: return launch_persistent_s<P, 1, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
+        case 8: return launch_persistent_s<P, 8, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
+        case 16: return launch_persistent_s<P, 16, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
+        case 32: return launch_persistent_s<P, 32, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
         default: return -1;
     }
 }
@@ -176,13 +206,13 @@ static int launch_persistent_w(const SignArgs &a, const SlotRef &sr, size_t nslo
 #define DIL_DEFINE_PERSISTENT_LAUNCHER(MODE)                                                                       \
     template <>                                                                                                  \
     int launch_persistent_p<Params<MODE>>(const SignArgs &a, const SlotRef &sr, size_t nslots,                    \
-                                          int slots_per_block, int occupancy, int blocks, uint32_t *d_head,       \
-                                          hipStream_t st) {                                                      \
+                                          int slots_per_block, int occupancy, int blocks, int claim,              \
+                                          uint32_t *d_head, unsigned long long *ph, hipStream_t st) {            \
         using P = Params<MODE>;                                                                                  \
         switch (occupancy) {                                                                                     \
-            case 2: return launch_persistent_w<P, 2>(a, sr, nslots, slots_per_block, blocks, d_head, st);        \
-            case 3: return launch_persistent_w<P, 3>(a, sr, nslots, slots_per_block, blocks, d_head, st);        \
-            case 4: return launch_persistent_w<P, 4>(a, sr, nslots, slots_per_block, blocks, d_head, st);        \
+            case 2: return launch_persistent_w<P, 2>(a, sr, nslots, slots_per_block, blocks, claim, d_head, ph, st); \
+            case 3: return launch_persistent_w<P, 3>(a, sr, nslots, slots_per_block, blocks, claim, d_head, ph, st); \
+            case 4: return launch_persistent_w<P, 4>(a, sr, nslots, slots_per_block, blocks, claim, d_head, ph, st); \
             default: return -1;                                                                                  \
         }                                                                                                        \
     }

@@ -184,6 +184,12 @@ __device__ __forceinline__ void stage_w(const SignArgs &a, size_t item, uint8_t 
     using Z = Sizes<P>;
     using SL = SlotLayout<P>;
     const ItemView<P> it(a, item);
+    // A rows are streamed from HBM with the next row's loads issued before
+    // the current row's products and INTT (software prefetch: the wave has
+    // an INTT's worth of work to cover each row's load latency)
+    int32_t an[P::L][4];
+#pragma unroll
+    for (int l = 0; l < P::L; ++l) wave_load_D(an[l], it.mat + l * N);
     int32_t yh[P::L][4];
 #pragma unroll
     for (int l = 0; l < P::L; ++l) {
@@ -192,12 +198,19 @@ __device__ __forceinline__ void stage_w(const SignArgs &a, size_t item, uint8_t 
     }
     for (int k = 0; k < P::K; ++k) {
         int32_t acc[4] = {0, 0, 0, 0};
+        int32_t ac[P::L][4];
+#pragma unroll
+        for (int l = 0; l < P::L; ++l)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ac[l][r] = an[l][r];
+        if (k + 1 < P::K) {
+#pragma unroll
+            for (int l = 0; l < P::L; ++l) wave_load_D(an[l], it.mat + ((k + 1) * P::L + l) * N);
+        }
 #pragma unroll
         for (int l = 0; l < P::L; ++l) {
-            int32_t av[4];
-            wave_load_D(av, it.mat + (k * P::L + l) * N);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc[r] += mont_mul(av[r], yh[l][r]);
+            for (int r = 0; r < 4; ++r) acc[r] += mont_mul(ac[l][r], yh[l][r]);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[r] = reduce32(acc[r]);
@@ -256,11 +269,25 @@ __device__ __forceinline__ bool stage_z(const SignArgs &a, size_t item, uint8_t 
 
     uint8_t *h = sig_out + Z::SIG_H;
     int pos = 0;
+    // operands of row k + 1 are loaded while row k is transformed (prefetch)
+    const int32_t *wbase = reinterpret_cast<const int32_t *>(slot + SL::W);
+    int32_t nr0[4], nct0[4], nw[4];
+    wave_load_D(nr0, it.s2h);
+    wave_load_D(nct0, it.t0h);
+    wave_load_A(nw, wbase);
     for (int k = 0; k < P::K; ++k) {
         int32_t r0[4], ct0[4], w[4];
-        wave_load_D(r0, it.s2h + k * N);
-        wave_load_D(ct0, it.t0h + k * N);
-        wave_load_A(w, reinterpret_cast<const int32_t *>(slot + SL::W) + k * N);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            r0[r] = nr0[r];
+            ct0[r] = nct0[r];
+            w[r] = nw[r];
+        }
+        if (k + 1 < P::K) {
+            wave_load_D(nr0, it.s2h + (k + 1) * N);
+            wave_load_D(nct0, it.t0h + (k + 1) * N);
+            wave_load_A(nw, wbase + (k + 1) * N);
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             r0[r] = mont_mul(ch[r], r0[r]);

@@ -196,6 +196,11 @@ int dil_sign_ctx_destroy(dil_sign_ctx *ctx);
  * 32), register budget as waves per SIMD (0 = automatic, or 2, 3, 4) and
  * workgroup count (0 = nslots / slots_per_block). */
 int dil_sign_ctx_set_tiling(dil_sign_ctx *ctx, int slots_per_block, int occupancy, int blocks);
+/* Persistent engine: at most items_per_block items owned by a workgroup at a
+ * time (0 = slots per workgroup).  Below that, each owned item gets several
+ * slots (consecutive nonces) per round, so small batches finish in fewer
+ * rounds at the price of speculative attempts; results are unchanged. */
+int dil_sign_ctx_set_claim(dil_sign_ctx *ctx, int items_per_block);
 /* Records HIP events on the call's stream around the fixed stage and the
  * rejection loop of the next max_calls dil_sign_device calls (0 turns timing
  * off and forgets earlier records).  dil_sign_ctx_timing waits for call
@@ -221,6 +226,12 @@ int dil_sign_ctx_status(dil_sign_ctx *ctx, int *status, void *stream);
 /* Attempts the last persistent-engine call executed, speculative ones
  * included (synchronises `stream`). */
 int dil_sign_ctx_executed_attempts(dil_sign_ctx *ctx, uint64_t *count, void *stream);
+/* Phase profile of the persistent engine (for tuning): when enabled, each
+ * workgroup sums shader-clock cycles spent in its six phases (assign, y, w,
+ * c, z, commit) and the sums over workgroups of the last call are returned by
+ * dil_sign_ctx_phase_cycles (synchronises `stream`).  Off by default. */
+int dil_sign_ctx_profile_phases(dil_sign_ctx *ctx, int enable);
+int dil_sign_ctx_phase_cycles(dil_sign_ctx *ctx, uint64_t *cycles6, void *stream);
 
 /* Resident keys (no counterpart in the reference, which re-derives A from
  * rho on every signature and verification, sign.cu:98-163 and

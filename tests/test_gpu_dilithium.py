@@ -191,3 +191,22 @@ def test_dropin_persistent_matches_perstage_full_rounds():
     b = d.sign(m, sk, exec_threshold=128, engine=ENGINE_PERSTAGE)
     assert np.array_equal(a, b)
     assert np.array_equal(a, oracle.sign_batch(2, m, sk)[0])
+
+
+@pytest.mark.parametrize("claim", [1, 3, 8])
+def test_persistent_engine_claim_cap(claim):
+    """Fewer items per workgroup than slots (several nonces per item per
+    round) changes no signature or attempt count."""
+    from cudilithium_amd import api
+    mode, n = 3, 150
+    _, sk = oracle.keypair_counter_batch(mode, n, rand_index=4000 + claim)
+    m = [bytes([claim]) * (i % 97) for i in range(n)]
+    exp, exp_att = oracle.sign_batch(mode, m, sk)
+    ctx = api.SignContext(mode, n, 256)
+    ctx.set_tiling(8, 2)
+    ctx.set_claim(claim)
+    ctx.profile_phases(True)
+    got = ctx.sign(m, sk)
+    assert np.array_equal(got, exp)
+    assert np.array_equal(ctx.attempt_counts(n), exp_att)
+    assert sum(ctx.phase_cycles().values()) > 0
