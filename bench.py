@@ -24,6 +24,15 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
+# The serving loop runs each (op, mode, message class) lane on its own stream
+# (15 streams); HIP maps streams onto at most GPU_MAX_HW_QUEUES hardware
+# queues (default 4) and streams that share a queue serialise -- a short
+# message's batch then waits behind a long message's hash.  Measured: with 16
+# queues lanes still shared queues, with 24-32 they did not.  Read by the HIP
+# runtime at initialisation, so it is set before anything loads it (an
+# explicit setting in the environment wins).
+if "--workload" in sys.argv and "serve" in sys.argv:
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "32")
 
 METRIC = "signs/s & verifies/s per MI355X (Dil2/3/5), bit-exact; serving p50/p99 latency"
 PEAK_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # CUs x SIMDs x lanes x GHz = 39.3 T int32 ops/s
@@ -66,6 +75,8 @@ def bench_sign(args, mode: int):
     nslots = ctx.nslots
     if args.spb or args.occupancy:
         ctx.set_tiling(args.spb, args.occupancy)
+    if args.claim:
+        ctx.set_claim(args.claim)
 
     for _ in range(args.warmup):
         ctx.sign_device(d_sig, d_sk, d_msgs, api.ENGINE_PERSISTENT)
@@ -151,6 +162,7 @@ def bench_sign(args, mode: int):
     cfg = {"workload": f"dilithium{mode}_sign_{n}_msg_loguniform_32B-4KiB",
            "batch": n, "mode": mode, "engine": "persistent+graph", "slots": nslots,
            "slots_per_block": ctx.slots_per_block or "auto", "occupancy": ctx.occupancy or "auto",
+           "claim": ctx.claim or "slots",
            "mean_attempts": float(attempts.mean()), "loop_phase_shares": phase_share}
     return {
         "metric": METRIC, "value": value, "unit": "signs/s", "ms_per_step": wall / args.steps * 1e3,
@@ -289,7 +301,7 @@ def bench_serve(args):
     keys = workloads.config5_keys(n_keys)
     caps = tuple(len(keys[m][0]) for m in workloads.SERVE_MODES)
     srv = api.Server(budget, lanes=args.lanes, max_batch=args.max_batch, max_keys=caps,
-                     batch_window_us=args.window_us)
+                     batch_window_us=args.window_us, long_message_bytes=args.long_bytes)
     for m in workloads.SERVE_MODES:
         srv.stores[m].keygen(0, keys[m][1])
     st0 = srv.stats()
@@ -308,16 +320,43 @@ def bench_serve(args):
                     sigs[i] = sg[j]
         return c, api.RequestBatch(c["ops"], c["modes"], c["keys"], c["msgs"], sigs, c["seeds"])
 
+    prev = [srv.stats()]
+
+    def op_breakdown():
+        st, p0 = srv.stats(), prev[0]
+        prev[0] = st
+        out = {}
+        for op, name in ((0, "keygen"), (1, "sign"), (2, "verify")):
+            nb = st.batches_by_op[op] - p0.batches_by_op[op]
+            if nb:
+                out[name] = {"batches": nb,
+                             "items_per_batch": (st.items_by_op[op] - p0.items_by_op[op]) / nb,
+                             "stage_us": (st.stage_ns[op] - p0.stage_ns[op]) / nb / 1e3,
+                             "device_us": (st.device_ns[op] - p0.device_ns[op]) / nb / 1e3,
+                             "finish_us": (st.finish_ns[op] - p0.finish_ns[op]) / nb / 1e3}
+        return out
+
     def summarize(b, c):
         sub, done = b.field("t_submit_ns").astype(np.int64), b.field("t_done_ns").astype(np.int64)
         lat = (done - sub) / 1e3
         status = b.field("status")
+        launch = b.field("t_launch_ns").astype(np.int64)
+        wait, service = (launch - sub) / 1e3, (done - launch) / 1e3
         out = {"requests": n, "errors": int((status != 0).sum()),
                "throughput_rps": n / ((done.max() - sub.min()) / 1e9),
-               "p50_us": _pct(lat, 50), "p99_us": _pct(lat, 99), "mean_us": float(lat.mean())}
+               "p50_us": _pct(lat, 50), "p99_us": _pct(lat, 99), "mean_us": float(lat.mean()),
+               "p50_wait_us": _pct(wait, 50), "p50_service_us": _pct(service, 50),
+               "p99_service_us": _pct(service, 99)}
         for op, name in ((0, "keygen"), (1, "sign"), (2, "verify")):
             sel = c["ops"] == op
             out[f"p99_us_{name}"] = _pct(lat[sel], 99)
+        lens = np.array([len(m) for m in c["msgs"]])
+        long_sel = (c["ops"] != api.OP_KEYGEN) & (lens > args.long_bytes)
+        out["p50_us_short"], out["p99_us_short"] = _pct(lat[~long_sel], 50), _pct(lat[~long_sel], 99)
+        out["p50_us_long"], out["p99_us_long"] = _pct(lat[long_sel], 50), _pct(lat[long_sel], 99)
+        out["long_fraction"] = float(long_sel.mean())
+        out["per_op_batch_means"] = op_breakdown()
+        log(f"[serve]   per-op batch means: {json.dumps(out['per_op_batch_means'])}")
         return out
 
     c, b = make(1e9, 0)
@@ -326,16 +365,22 @@ def bench_serve(args):
     sat = summarize(b, c)
     log(f"[serve] saturation {sat['throughput_rps']:.0f} req/s")
     runs = []
-    for k, frac in enumerate((0.3, 0.6, 0.9)):
-        c, b = make(frac * sat["throughput_rps"], k + 1)
+    # 30 / 60 / 90 % of the burst saturation, then fixed absolute rates
+    levels = [(frac * sat["throughput_rps"], frac) for frac in args.fractions]
+    levels += [(float(r), None) for r in args.rates]
+    for k, (rate, frac) in enumerate(levels):
+        c, b = make(rate, k + 1)
         srv.replay(b.reqs, c["arrival_ns"])
         srv.drain()
         r = summarize(b, c)
-        r["offered_rps"] = frac * sat["throughput_rps"]
+        r["offered_rps"] = rate
         r["offered_fraction"] = frac
         runs.append(r)
         log(f"[serve] offered {r['offered_rps']:.0f} req/s: p50 {r['p50_us']:.0f} us, p99 {r['p99_us']:.0f} us, "
-            f"{r['throughput_rps']:.0f} req/s")
+            f"{r['throughput_rps']:.0f} req/s (p50 queue wait {r['p50_wait_us']:.0f} us, p50 / p99 service "
+            f"{r['p50_service_us']:.0f} / {r['p99_service_us']:.0f} us; messages <= {args.long_bytes} B p50 / p99 "
+            f"{r['p50_us_short']:.0f} / {r['p99_us_short']:.0f} us, longer {r['p50_us_long']:.0f} / "
+            f"{r['p99_us_long']:.0f} us)")
 
     # parity of the last run on a sample, and isolation (alone vs in the mix)
     import oracle
@@ -373,7 +418,8 @@ def bench_serve(args):
             "dtype": "int32", "parity": "bit-exact" if (ok and alone_ok) else "MISMATCH",
             "config": {"workload": f"serving_mix_{n_keys}_resident_keys_keygen20_sign50_verify30_msg_loguniform_32B-64KiB",
                        "requests_per_run": n, "hbm_budget_bytes": budget, "lanes": args.lanes,
-                       "max_batch": args.max_batch, "batch_window_us": args.window_us, "batches": int(st.batches)},
+                       "max_batch": args.max_batch, "batch_window_us": args.window_us,
+                       "long_message_bytes": args.long_bytes, "batches": int(st.batches)},
             "saturation": sat, "loads": runs, "roofline": None, "cpu_baseline": None}
 
 
@@ -415,15 +461,21 @@ def main():
     ap.add_argument("--slots", type=int, default=0, help="attempt slots (default: batch)")
     ap.add_argument("--spb", type=int, default=0, help="slots per workgroup (0 = autotuned)")
     ap.add_argument("--occupancy", type=int, default=0, help="waves per SIMD of the loop kernel (0 = autotuned)")
+    ap.add_argument("--claim", type=int, default=0, help="items per workgroup of the loop kernel (0 = its slots)")
     ap.add_argument("--check", type=int, default=128, help="items checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perstage", action="store_true")
     ap.add_argument("--requests", type=int, default=40000, help="serve: requests per run")
-    ap.add_argument("--budget-gib", type=float, default=6.0, help="serve: HBM budget of the in-flight batches")
-    ap.add_argument("--lanes", type=int, default=3, help="serve: batches in flight")
+    ap.add_argument("--budget-gib", type=float, default=4.0, help="serve: HBM budget of the in-flight batches")
+    ap.add_argument("--lanes", type=int, default=1, help="serve: batches in flight per (op, mode) group")
     ap.add_argument("--max-batch", type=int, default=4096, help="serve: requests per batch")
     ap.add_argument("--window-us", type=int, default=0, help="serve: batching window")
+    ap.add_argument("--long-bytes", type=int, default=4096, help="serve: messages above this get lanes of their own")
+    ap.add_argument("--fractions", type=float, nargs="*", default=[0.3, 0.6, 0.9],
+                    help="serve: offered loads as fractions of the burst saturation")
+    ap.add_argument("--rates", type=float, nargs="*", default=[25000, 50000, 100000],
+                    help="serve: fixed offered loads (requests/s) measured after the saturation fractions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "

@@ -295,9 +295,12 @@ class SignContext:
         self.handle = p.value
         if tuned and nslots // tuned[1] >= 1:
             self.set_tiling(tuned[1], tuned[2])
+            if tuned[3]:
+                self.set_claim(tuned[3])
 
     slots_per_block = 0
     occupancy = 0
+    claim = 0
 
     def set_tiling(self, slots_per_block: int = 0, occupancy: int = 0, blocks: int = 0) -> None:
         _lib.check(L().dil_sign_ctx_set_tiling(self.handle, slots_per_block, occupancy, blocks), "set_tiling")
@@ -305,6 +308,7 @@ class SignContext:
 
     def set_claim(self, items_per_block: int) -> None:
         _lib.check(L().dil_sign_ctx_set_claim(self.handle, int(items_per_block)), "set_claim")
+        self.claim = int(items_per_block)
 
     def enable_timing(self, max_calls: int) -> None:
         _lib.check(L().dil_sign_ctx_enable_timing(self.handle, int(max_calls)), "enable_timing")
@@ -518,7 +522,8 @@ OP_KEYGEN, OP_SIGN, OP_VERIFY = 0, 1, 2
 
 class ServerConfig(ctypes.Structure):  # dil_server_config
     _fields_ = [("hbm_budget_bytes", c_size_t), ("lanes", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
-                ("max_keys", ctypes.c_uint32 * 3), ("batch_window_us", ctypes.c_uint32)]
+                ("max_keys", ctypes.c_uint32 * 3), ("batch_window_us", ctypes.c_uint32),
+                ("long_message_bytes", ctypes.c_uint32)]
 
 
 class Request(ctypes.Structure):  # dil_request
@@ -532,7 +537,9 @@ class Request(ctypes.Structure):  # dil_request
 class ServerStats(ctypes.Structure):  # dil_server_stats
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
                 ("rejected", ctypes.c_uint64), ("items_by_op", ctypes.c_uint64 * 3), ("lane_bytes", ctypes.c_uint64),
-                ("max_msg_bytes", ctypes.c_uint64)]
+                ("max_msg_bytes", ctypes.c_uint64), ("batches_by_op", ctypes.c_uint64 * 3),
+                ("stage_ns", ctypes.c_uint64 * 3), ("device_ns", ctypes.c_uint64 * 3),
+                ("finish_ns", ctypes.c_uint64 * 3)]
 
 
 def _register_server():
@@ -565,11 +572,12 @@ class Server:
     resident keys.  Requests are `Request` structs whose buffers the caller
     keeps alive until done (Server.requests builds such arrays)."""
 
-    def __init__(self, hbm_budget_bytes: int, lanes: int = 3, max_batch: int = 4096, max_keys=(0, 0, 0),
-                 batch_window_us: int = 0):
+    def __init__(self, hbm_budget_bytes: int, lanes: int = 1, max_batch: int = 4096, max_keys=(0, 0, 0),
+                 batch_window_us: int = 0, long_message_bytes: int = 4096):
         L()
         _register_server()
-        cfg = ServerConfig(hbm_budget_bytes, lanes, max_batch, (ctypes.c_uint32 * 3)(*max_keys), batch_window_us)
+        cfg = ServerConfig(hbm_budget_bytes, lanes, max_batch, (ctypes.c_uint32 * 3)(*max_keys), batch_window_us,
+                           long_message_bytes)
         p = c_void_p()
         _lib.check(L().dil_server_create(ctypes.byref(p), ctypes.byref(cfg)), "server_create")
         self.handle = p.value

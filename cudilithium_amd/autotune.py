@@ -1,8 +1,10 @@
 """Autotuner of the persistent signing engine.
 
 For a (mode, batch-size bucket) it times the engine over candidate tilings --
-attempt slots (a multiple of the batch) and slots per workgroup -- on
-synthetic inputs of that shape, and records the fastest in
+attempt slots (a multiple of the batch), slots per workgroup, register budget
+(waves per SIMD) and items claimed per workgroup (all slots, or fewer so that
+each item gets several nonces per round) -- on synthetic inputs of that shape,
+and records the fastest in
 cudilithium_amd/tuning.json.  SignContext looks the bucket up when the caller
 does not choose a tiling.
 
@@ -23,6 +25,7 @@ CACHE = os.path.join(HERE, "tuning.json")
 SLOT_FACTORS = (0.5, 1.0, 1.5, 2.0)
 SLOTS_PER_BLOCK = (8, 16, 32)
 OCCUPANCY = (2, 3, 4)
+CLAIM_DIVISORS = (1, 2, 4)  # claim = slots per workgroup / divisor
 
 
 def bucket(n: int) -> int:
@@ -41,14 +44,14 @@ def load_cache() -> dict:
 
 
 def lookup(mode: int, batch: int):
-    """(slot_factor, slots_per_block, occupancy) recorded for this mode and
-    batch bucket, or None."""
+    """(slot_factor, slots_per_block, occupancy, claim) recorded for this mode
+    and batch bucket, or None."""
     e = load_cache().get(f"sign{mode}:{bucket(batch)}")
-    return (e["slot_factor"], e["slots_per_block"], e.get("occupancy", 0)) if e else None
+    return (e["slot_factor"], e["slots_per_block"], e.get("occupancy", 0), e.get("claim", 0)) if e else None
 
 
 def tune_sign(mode: int, batch: int, reps: int = 3, factors=SLOT_FACTORS, spbs=SLOTS_PER_BLOCK, occs=OCCUPANCY,
-              log=print):
+              claims=CLAIM_DIVISORS, log=print):
     from . import api, workloads
     from .device import DeviceBuffer, synchronize
 
@@ -64,8 +67,10 @@ def tune_sign(mode: int, batch: int, reps: int = 3, factors=SLOT_FACTORS, spbs=S
         for spb in spbs:
             if nslots // spb < 1:
                 continue
-            for occ in occs:
+            for occ, div in ((o, d) for o in occs for d in claims):
+                claim = max(1, spb // div) if div > 1 else 0
                 ctx.set_tiling(spb, occ)
+                ctx.set_claim(claim)
                 ctx.sign_device(d_sig, d_sk, dm)
                 synchronize()
                 ctx.enable_timing(reps)
@@ -78,7 +83,7 @@ def tune_sign(mode: int, batch: int, reps: int = 3, factors=SLOT_FACTORS, spbs=S
                 ctx.enable_timing(0)
                 executed = ctx.executed_attempts()
                 needed = int(ctx.attempt_counts(batch).sum())
-                r = {"slot_factor": f, "nslots": nslots, "slots_per_block": spb, "occupancy": occ,
+                r = {"slot_factor": f, "nslots": nslots, "slots_per_block": spb, "occupancy": occ, "claim": claim,
                      "blocks": nslots // spb, "batch_ms": wall * 1e3, "loop_ms": loop,
                      "executed_attempts": executed, "needed_attempts": needed,
                      "speculative_overhead": executed / max(1, needed) - 1}
@@ -94,15 +99,17 @@ def main():
     ap.add_argument("--mode", type=int, default=3)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--occupancy", type=int, nargs="*", default=list(OCCUPANCY))
     ap.add_argument("--write", action="store_true", help="record the best tiling in tuning.json")
     args = ap.parse_args()
-    best, _ = tune_sign(args.mode, args.batch, args.reps, log=lambda s: print(s, file=sys.stderr, flush=True))
+    best, _ = tune_sign(args.mode, args.batch, args.reps, occs=tuple(args.occupancy),
+                        log=lambda s: print(s, file=sys.stderr, flush=True))
     print(json.dumps({"best": best}))
     if args.write:
         cache = load_cache()
         cache[f"sign{args.mode}:{bucket(args.batch)}"] = {
             "slot_factor": best["slot_factor"], "slots_per_block": best["slots_per_block"],
-            "occupancy": best["occupancy"], "batch_ms": best["batch_ms"]}
+            "occupancy": best["occupancy"], "claim": best["claim"], "batch_ms": best["batch_ms"]}
         with open(CACHE, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
 

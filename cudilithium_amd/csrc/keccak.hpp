@@ -177,12 +177,23 @@ __device__ __forceinline__ void shake_absorb(Lane64 (&s)[25], Get get, uint32_t 
 #pragma unroll
     for (int i = 0; i < 25; ++i) s[i] = {0u, 0u};
     uint32_t base = 0;
+    // full blocks, the next block's words loaded before the current block's
+    // permutation so that their memory latency overlaps it
+    uint64_t cur[RW];
+    if (len >= (uint32_t)RATE) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i) cur[i] = get(i, 8u);
+    }
     while (len >= (uint32_t)RATE) {
 #pragma unroll
-        for (int i = 0; i < RW; ++i) xor_word(s[i], get(base + i, 8u));
-        keccak_f1600(s);
+        for (int i = 0; i < RW; ++i) xor_word(s[i], cur[i]);
         base += RW;
         len -= RATE;
+        if (len >= (uint32_t)RATE) {
+#pragma unroll
+            for (int i = 0; i < RW; ++i) cur[i] = get(base + i, 8u);
+        }
+        keccak_f1600(s);
     }
     const uint32_t nw = len >> 3, rem = len & 7;
 #pragma unroll

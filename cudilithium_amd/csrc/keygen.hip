@@ -199,9 +199,10 @@ static int keygen_impl(const KeygenArgs &a, hipStream_t st) {
     hipLaunchKernelGGL(kg_seed_kernel<P>, dim3(b1), dim3(256), 0, st, a);
     hipLaunchKernelGGL(kg_eta_kernel<P>, dim3((unsigned)((a.n * NP + 255) / 256)), dim3(256), 0, st, a);
     const size_t smem = sizeof(KgShared<P>);
-    hipError_t e = hipFuncSetAttribute((const void *)kg_main_kernel<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)smem);
-    if (e != hipSuccess) return (int)e;
+    // set once per kernel: a host call, not a stream operation
+    static const hipError_t attr = hipFuncSetAttribute((const void *)kg_main_kernel<P>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (attr != hipSuccess) return (int)attr;
     hipLaunchKernelGGL(kg_main_kernel<P>, dim3((unsigned)((a.n + G - 1) / G)), dim3(256), smem, st, a);
     hipLaunchKernelGGL(kg_tr_kernel<P>, dim3(b1), dim3(256), 0, st, a);
     return (int)hipGetLastError();

@@ -126,11 +126,14 @@ __device__ __forceinline__ void mask_raw(const uint64_t (&rp)[8], uint32_t nonce
 // memory, so the byte loop has no dependent loads (the dense in-memory form
 // pays a memory round trip per step).  i >= N - TAU >= 192 lies in words
 // 6..7.  The masks are stored as nz[0..7] | neg[0..7] (64 bytes).
+// word w of a (w < 8) by a select tree on the bits of w: no indexed access
+// that the compiler could lower to scratch memory
 __device__ __forceinline__ uint32_t mask_word(const uint32_t (&a)[8], uint32_t w) {
-    uint32_t r = a[0];
-#pragma unroll
-    for (int k = 1; k < 8; ++k) r = w == (uint32_t)k ? a[k] : r;
-    return r;
+    const bool b0 = w & 1u, b1 = w & 2u, b2 = w & 4u;
+    const uint32_t t0 = b0 ? a[1] : a[0], t1 = b0 ? a[3] : a[2];
+    const uint32_t t2 = b0 ? a[5] : a[4], t3 = b0 ? a[7] : a[6];
+    const uint32_t u0 = b1 ? t1 : t0, u1 = b1 ? t3 : t2;
+    return b2 ? u1 : u0;
 }
 
 template <int TAU>
@@ -155,13 +158,13 @@ __device__ __forceinline__ void challenge(const uint64_t (&ct)[4], uint32_t *mas
             nz[7] = hi7 ? ((nz[7] & ~mi) | nzb) : nz[7];
             neg[6] = hi7 ? neg[6] : ((neg[6] & ~mi) | ngb);
             neg[7] = hi7 ? ((neg[7] & ~mi) | ngb) : neg[7];
-            // c[b] = (-1)^(sign bit)
-            const uint32_t sb = (uint32_t)signs & 1u ? mb : 0u;
+            // c[b] = (-1)^(sign bit); word wb selected by a one-hot mask
+            const uint32_t sb = (uint32_t)signs & 1u ? mb : 0u, onehot = 1u << wb;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                const bool here = wb == (uint32_t)k;
-                nz[k] = here ? (nz[k] | mb) : nz[k];
-                neg[k] = here ? ((neg[k] & ~mb) | sb) : neg[k];
+                const uint32_t here = 0u - ((onehot >> k) & 1u);
+                nz[k] |= mb & here;
+                neg[k] = (neg[k] & ~(mb & here)) | (sb & here);
             }
             signs >>= 1;
             ++i;

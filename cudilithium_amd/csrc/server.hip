@@ -1,23 +1,35 @@
 // Serving loop: continuous batching of single keygen / sign / verify
-// requests onto in-flight batches ("lanes") under a fixed HBM budget.
+// requests under a fixed HBM budget.
 //
 // Requests arrive from any host thread (dil_server_submit) into one FIFO
-// per (op, mode).  One worker thread owns the device:
-//   harvest  a lane whose completion event has fired scatters its outputs
-//            into the requests (signature bytes, verdict, pk || sk), stamps
-//            t_done and publishes done = 1
-//   launch   while a lane is free and requests wait, a batch takes the
-//            oldest requests across all (op, mode) groups, up to max_batch
-//            and the lane's message area; inputs are staged into the lane's
-//            pinned buffer and copied in one transfer, each group's pipeline
-//            runs on its own stream (forked from the lane's stream and joined
-//            back), the outputs come back in one transfer and an event marks
-//            completion
-// Each lane holds everything one batch needs (inputs, outputs, verify /
-// keygen work rows and one resident-only signing context per mode), carved
-// from the budget at creation, so admission never allocates.  Sign and
-// verify requests name resident keys (dil_keystore, one per mode), whose
-// expanded A, s1^, s2^, t0^, t1^ and tr are not recomputed per request.
+// per group, a group being an (op, mode) pair (9 groups).  Each group owns
+// `lanes` lanes; a lane is one batch in flight with its own stream, pinned
+// staging buffers and device buffers sized for its group's op.  One worker
+// thread per lane runs the lane's cycle:
+//   launch   when its group has waiting requests, take them in FIFO order,
+//            up to max_batch and the lane's message area, stage the inputs
+//            into the pinned buffer, copy them in one transfer, enqueue the
+//            group's pipeline (persistent sign with resident keys / resident
+//            verify / keygen) on the lane's stream, copy the outputs back in
+//            one transfer and record an event
+//   harvest  wait for the event (blocking in the driver), scatter the
+//            outputs into the requests (signature bytes, verdict, pk || sk),
+//            stamp t_done and publish done = 1
+// Lanes never wait for each other's host work.
+// Groups run concurrently on their own streams, so a request waits at most
+// for its own group's batch in flight (with lanes = 1), not for other
+// groups'.  A batch lasts as long as its longest message takes to hash (one
+// lane absorbs a message serially, ~10 us per 136-byte block), so sign and
+// verify groups keep messages longer than long_message_bytes in queues and
+// lanes of their own: short requests are never held up by a long one.  Requests that arrive while batches run join the next batch of
+// their group as soon as a lane frees (continuous batching).  All device
+// memory -- every lane's buffers and signing context and the message areas
+// -- is carved from the budget at creation, so admission never allocates; a
+// message larger than its group's message area is rejected (DIL_ERANGE).
+// Streams that the HIP runtime maps onto one hardware queue serialise; it
+// has at most GPU_MAX_HW_QUEUES of them (default 4), and with as many queues
+// as streams lanes were still seen sharing queues, so a server with 15 lanes
+// wants 24-32.
 #include <hip/hip_runtime.h>
 #include <time.h>
 
@@ -27,6 +39,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -47,31 +60,26 @@ uint64_t mono_ns() {
 
 int mode_slot(int mode) { return mode == 2 ? 0 : (mode == 3 ? 1 : (mode == 5 ? 2 : -1)); }
 constexpr int MODES[3] = {2, 3, 5};
+constexpr int NGROUPS = 9;  // group g = 3 * op + mode slot
 
 constexpr size_t al8(size_t x) { return (x + 7) & ~size_t(7); }
 constexpr size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
 
-// per-item byte maxima over the three modes
-constexpr size_t MAX_SIG = Sizes<Params<5>>::SIG_BYTES;
-constexpr size_t MAX_KEYPAIR = Sizes<Params<5>>::PK_BYTES + Sizes<Params<5>>::SK_BYTES;
+size_t out_item_bytes(int op, int mode) {
+    if (op == DIL_OP_SIGN) return dil_signature_bytes(mode);
+    if (op == DIL_OP_VERIFY) return 4;
+    return dil_public_key_bytes(mode) + dil_secret_key_bytes(mode);
+}
 
-// groups g = 3 * op + mode slot; a lane's batch holds any mix of them
-struct Lane {
-    hipStream_t st = nullptr;  // copies, fork and join
-    hipStream_t sub[9] = {};   // one stream per group's pipeline
-    hipEvent_t ev = nullptr, fork = nullptr, join[9] = {};
-    uint8_t *h_in = nullptr, *h_out = nullptr;
-    uint8_t *d_in = nullptr, *d_out = nullptr, *d_work = nullptr;
-    dil_sign_ctx *ctx[3] = {nullptr, nullptr, nullptr};
-    std::vector<dil_request *> reqs;  // ordered by group
-    size_t g0[10] = {};               // group g is reqs[g0[g], g0[g + 1])
-    size_t out_off[9] = {};           // group g's outputs in h_out / d_out
-    bool busy = false;
-    uint32_t batch_id = 0;
-};
+// fixed-size input per item: the signature to verify, or the keygen seed
+size_t payload_item_bytes(int op, int mode) {
+    if (op == DIL_OP_VERIFY) return dil_signature_bytes(mode);
+    if (op == DIL_OP_KEYGEN) return 32;
+    return 0;
+}
 
 // input area of a batch of n: key_idx u32[n] | moff u64[n] | mlen u32[n] |
-// fixed-size payload (verify: signatures, keygen: seeds) | messages
+// payload (n fixed-size items) | messages
 struct InLayout {
     size_t idx, moff, mlen, payload, msgs;
     InLayout(size_t n, size_t payload_bytes) {
@@ -83,6 +91,54 @@ struct InLayout {
     }
 };
 
+size_t sign_ctx_bytes(int mode, size_t b) {
+    const size_t row = mode == 2 ? al256(SignLayout<Params<2>>::ints(0))
+                                 : (mode == 3 ? al256(SignLayout<Params<3>>::ints(0))
+                                              : al256(SignLayout<Params<5>>::ints(0)));
+    return b * (row + al256(dil_sign_temp_row_bytes(mode)));
+}
+
+struct Lane {
+    int group = 0, op = 0, mode = 0, cls = 0;  // cls 1: messages above long_message_bytes
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr, ev0 = nullptr;  // end (completion), start (timing)
+    uint8_t *h_in = nullptr, *h_out = nullptr;
+    uint8_t *d_in = nullptr, *d_out = nullptr, *d_work = nullptr;
+    size_t in_fixed = 0, msg_cap = 0, work_pitch = 0;
+    dil_sign_ctx *ctx = nullptr;  // sign lanes
+    std::vector<dil_request *> reqs;
+    uint32_t batch_id = 0;
+    bool backlog = false;  // requests of the group were still waiting after fill()
+    std::condition_variable *cv = nullptr;  // woken when its queue gets work
+};
+
+// A stream with a hardware queue of its own.  The HIP runtime may map several
+// streams created while the device is idle onto one hardware queue, where
+// their work serialises (a short message's batch then waits behind a long
+// message's hash); a stream created with a CU mask always gets its own
+// queue, and the mask here enables every CU, so nothing else changes.
+int create_lane_stream(hipStream_t *st) {
+    int cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return (int)hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+    std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0xFFFFFFFFu);
+    if (cus % 32) mask.back() = (1u << (cus % 32)) - 1u;
+    return (int)hipExtStreamCreateWithCUMask(st, (uint32_t)mask.size(), mask.data());
+}
+
+void free_lane(Lane &ln) {
+    if (ln.ctx) dil_sign_ctx_destroy(ln.ctx);
+    (void)hipFree(ln.d_in);
+    (void)hipFree(ln.d_out);
+    (void)hipFree(ln.d_work);
+    (void)hipHostFree(ln.h_in);
+    (void)hipHostFree(ln.h_out);
+    if (ln.ev) (void)hipEventDestroy(ln.ev);
+    if (ln.ev0) (void)hipEventDestroy(ln.ev0);
+    if (ln.st) (void)hipStreamDestroy(ln.st);
+}
+
 }  // namespace
 
 struct dil_server {
@@ -90,23 +146,28 @@ struct dil_server {
     int device = 0;
     dil_keystore *ks[3] = {nullptr, nullptr, nullptr};
     std::vector<Lane> lanes;
-    size_t in_fixed = 0, msg_cap = 0, out_cap = 0, work_pitch = 0, lane_bytes = 0;
+    size_t lane_bytes_max = 0, msg_cap = 0, long_bytes = 0;
 
     std::mutex mu;  // queues, stop flag, counters below
-    std::condition_variable cv_sub;
-    std::deque<dil_request *> q[3][3];  // [op][mode slot]
+    // one condition variable per lane; a submit wakes only the lanes of the
+    // queues it filled (waking every lane thread per request floods the CPUs)
+    std::vector<std::unique_ptr<std::condition_variable>> cvs;
+    std::vector<int> lanes_of[NGROUPS][2];
+    std::deque<dil_request *> q[NGROUPS][2];  // [group][short / long message]
     size_t npending = 0;
     bool stop = false;
     uint64_t submitted = 0, batches = 0, rejected = 0, by_op[3] = {0, 0, 0};
+    // per-op timing sums (written by the worker, read by dil_server_get_stats)
+    std::atomic<uint64_t> nb_op[3] = {}, stage_ns[3] = {}, device_ns[3] = {}, finish_ns[3] = {};
 
     std::mutex done_mu;
     std::condition_variable cv_done;
     std::atomic<uint64_t> completed{0};
 
-    std::thread worker;
+    std::vector<std::thread> workers;  // one per lane
     uint32_t next_batch = 0;
 
-    void run();
+    void lane_loop(Lane &ln);
     bool fill(Lane &ln);
     int enqueue(Lane &ln);
     void finish(Lane &ln, int err);
@@ -117,101 +178,44 @@ struct dil_server {
     }
 };
 
-namespace {
-
-void free_lane(Lane &ln) {
-    for (auto *c : ln.ctx)
-        if (c) dil_sign_ctx_destroy(c);
-    (void)hipFree(ln.d_in);
-    (void)hipFree(ln.d_out);
-    (void)hipFree(ln.d_work);
-    (void)hipHostFree(ln.h_in);
-    (void)hipHostFree(ln.h_out);
-    if (ln.ev) (void)hipEventDestroy(ln.ev);
-    if (ln.fork) (void)hipEventDestroy(ln.fork);
-    for (int g = 0; g < 9; ++g) {
-        if (ln.join[g]) (void)hipEventDestroy(ln.join[g]);
-        if (ln.sub[g]) (void)hipStreamDestroy(ln.sub[g]);
-    }
-    if (ln.st) (void)hipStreamDestroy(ln.st);
-}
-
-size_t sign_ctx_bytes(int mode, size_t b) {
-    const size_t row = mode == 2 ? al256(SignLayout<Params<2>>::ints(0))
-                                 : (mode == 3 ? al256(SignLayout<Params<3>>::ints(0))
-                                              : al256(SignLayout<Params<5>>::ints(0)));
-    return b * (row + al256(dil_sign_temp_row_bytes(mode)));
-}
-
-}  // namespace
-
-// Takes the next batch for a free lane (called with mu held): requests of
-// every (op, mode) group, oldest first across groups, up to max_batch and the
-// lane's message area; then ordered by group so that each group is one
-// contiguous range.  Returns false when nothing is ready.
+// Takes the next batch of the lane's group (called with mu held): FIFO, up
+// to max_batch and the lane's message area.  Returns false when nothing is
+// ready.
 bool dil_server::fill(Lane &ln) {
-    const uint64_t now = mono_ns();
-    uint64_t oldest = ~0ull;
-    size_t waiting = 0;
-    for (int o = 0; o < 3; ++o)
-        for (int m = 0; m < 3; ++m)
-            if (!q[o][m].empty()) {
-                oldest = std::min<uint64_t>(oldest, q[o][m].front()->t_submit_ns);
-                waiting += q[o][m].size();
-            }
-    if (waiting == 0) return false;
+    auto &dq = q[ln.group][ln.cls];
+    if (dq.empty()) return false;
     // hold a partial batch for the window (it fills while other lanes run)
-    if (cfg.batch_window_us && waiting < cfg.max_batch && now - oldest < 1000ull * cfg.batch_window_us) return false;
+    if (cfg.batch_window_us && dq.size() < cfg.max_batch &&
+        mono_ns() - dq.front()->t_submit_ns < 1000ull * cfg.batch_window_us)
+        return false;
     ln.reqs.clear();
     size_t bytes = 0;
-    for (;;) {
-        int bo = -1, bm = -1;
-        uint64_t t = ~0ull;
-        for (int o = 0; o < 3; ++o)
-            for (int m = 0; m < 3; ++m)
-                if (!q[o][m].empty() && q[o][m].front()->t_submit_ns < t) {
-                    t = q[o][m].front()->t_submit_ns;
-                    bo = o;
-                    bm = m;
-                }
-        if (bo < 0 || ln.reqs.size() >= cfg.max_batch) break;
-        dil_request *r = q[bo][bm].front();
-        const size_t need = bo == DIL_OP_KEYGEN ? 0 : al8(r->mlen);
-        if (bytes + need > msg_cap) break;
+    while (!dq.empty() && ln.reqs.size() < cfg.max_batch) {
+        dil_request *r = dq.front();
+        const size_t need = ln.op == DIL_OP_KEYGEN ? 0 : al8(r->mlen);
+        if (bytes + need > ln.msg_cap) break;
         bytes += need;
         ln.reqs.push_back(r);
-        q[bo][bm].pop_front();
+        dq.pop_front();
     }
-    std::stable_sort(ln.reqs.begin(), ln.reqs.end(), [](const dil_request *a, const dil_request *b) {
-        return a->op * 3 + mode_slot(a->mode) < b->op * 3 + mode_slot(b->mode);
-    });
-    for (int g = 0; g <= 9; ++g) ln.g0[g] = 0;
-    for (const auto *r : ln.reqs) ++ln.g0[r->op * 3 + mode_slot(r->mode) + 1];
-    for (int g = 0; g < 9; ++g) ln.g0[g + 1] += ln.g0[g];
     npending -= ln.reqs.size();
+    ln.backlog = !dq.empty();
     ln.batch_id = next_batch++;
     ++batches;
-    for (const auto *r : ln.reqs) ++by_op[r->op];
+    by_op[ln.op] += ln.reqs.size();
     return !ln.reqs.empty();
 }
 
-namespace {
-size_t out_item_bytes(int op, int mode) {
-    if (op == DIL_OP_SIGN) return dil_signature_bytes(mode);
-    if (op == DIL_OP_VERIFY) return 4;
-    return dil_public_key_bytes(mode) + dil_secret_key_bytes(mode);
-}
-}  // namespace
-
-// Stages inputs and enqueues every group's pipeline of the lane's batch, each
-// on its own stream forked from and joined back into the lane's stream.
+// Stages inputs and enqueues the group's pipeline for the lane's batch.
 int dil_server::enqueue(Lane &ln) {
     const size_t n = ln.reqs.size();
-    const InLayout L(n, n * MAX_SIG);
+    const int mode = ln.mode, ms = mode_slot(mode);
+    const size_t pkb = dil_public_key_bytes(mode), skb = dil_secret_key_bytes(mode);
+    const size_t sigb = dil_signature_bytes(mode), pay = payload_item_bytes(ln.op, mode);
+    const InLayout L(n, n * pay);
     uint32_t *idx = reinterpret_cast<uint32_t *>(ln.h_in + L.idx);
     uint64_t *moff = reinterpret_cast<uint64_t *>(ln.h_in + L.moff);
     uint32_t *mlen = reinterpret_cast<uint32_t *>(ln.h_in + L.mlen);
-    uint8_t *pay = ln.h_in + L.payload;
     size_t pos = 0;
     const uint64_t t = mono_ns();
     for (size_t i = 0; i < n; ++i) {
@@ -220,120 +224,97 @@ int dil_server::enqueue(Lane &ln) {
         r->batch_id = ln.batch_id;
         idx[i] = r->key;
         moff[i] = pos;
-        mlen[i] = r->op == DIL_OP_KEYGEN ? 0 : r->mlen;
-        if (r->op == DIL_OP_VERIFY) std::memcpy(pay + i * MAX_SIG, r->sig, dil_signature_bytes(r->mode));
-        if (r->op == DIL_OP_KEYGEN) std::memcpy(pay + i * MAX_SIG, r->seed, 32);
-        if (r->op != DIL_OP_KEYGEN && r->mlen) std::memcpy(ln.h_in + L.msgs + pos, r->msg, r->mlen);
-        if (r->op != DIL_OP_KEYGEN) pos += al8(r->mlen);
-    }
-    int e = (int)hipMemcpyAsync(ln.d_in, ln.h_in, L.msgs + std::max<size_t>(pos, 8), hipMemcpyHostToDevice, ln.st);
-    if (e) return e;
-    if ((e = (int)hipEventRecord(ln.fork, ln.st))) return e;
-    const uint8_t *d_msg = ln.d_in + L.msgs;
-    size_t out = 0;
-    for (int g = 0; g < 9; ++g) {
-        const size_t g0 = ln.g0[g], ng = ln.g0[g + 1] - g0;
-        ln.out_off[g] = out;
-        if (ng == 0) continue;
-        const int op = g / 3, ms = g % 3, mode = MODES[ms];
-        const size_t pkb = dil_public_key_bytes(mode), skb = dil_secret_key_bytes(mode);
-        const size_t sigb = dil_signature_bytes(mode);
-        hipStream_t gs = ln.sub[g];
-        if ((e = (int)hipStreamWaitEvent(gs, ln.fork, 0))) return e;
-        const uint32_t *d_idx = reinterpret_cast<const uint32_t *>(ln.d_in + L.idx) + g0;
-        const uint64_t *d_moff = reinterpret_cast<const uint64_t *>(ln.d_in + L.moff) + g0;
-        const uint32_t *d_mlen = reinterpret_cast<const uint32_t *>(ln.d_in + L.mlen) + g0;
-        const uint8_t *d_pay = ln.d_in + L.payload + g0 * MAX_SIG;
-        uint8_t *d_out = ln.d_out + out;
-        uint8_t *d_work = ln.d_work + g0 * work_pitch;
-        if (op == DIL_OP_SIGN) {
-            dil_sign_ctx *c = ln.ctx[ms];
-            // small batches: fewer slots per workgroup, spread over all of them,
-            // so that each item gets spb / claim nonces per round
-            int spb = ng >= 8192 ? 32 : (ng >= 2048 ? 16 : 8);
-            if ((size_t)spb > cfg.max_batch) spb = 1;
-            if ((e = dil_sign_ctx_set_tiling(c, spb, 0, 0))) return e;
-            const size_t blocks = cfg.max_batch / (size_t)spb;
-            const int claim = (int)std::min<size_t>((size_t)spb, std::max<size_t>(1, (ng + blocks - 1) / blocks));
-            if ((e = dil_sign_ctx_set_claim(c, claim))) return e;
-            e = dil_sign_device_resident(c, ks[ms], d_idx, d_out, sigb, d_msg, d_moff, d_mlen, ng,
-                                         DIL_ENGINE_PERSISTENT, gs);
-        } else if (op == DIL_OP_VERIFY) {
-            e = dil_verify_device_resident(ks[ms], d_idx, reinterpret_cast<int32_t *>(d_out), d_pay, MAX_SIG, d_msg,
-                                           d_moff, d_mlen, ng, d_work, work_pitch, gs);
-        } else {
-            e = dil_keypair_device(mode, d_out, pkb + skb, d_out + pkb, pkb + skb, d_pay, MAX_SIG, 0, ng, d_work,
-                                   work_pitch, gs);
+        mlen[i] = ln.op == DIL_OP_KEYGEN ? 0 : r->mlen;
+        if (ln.op == DIL_OP_VERIFY) std::memcpy(ln.h_in + L.payload + i * pay, r->sig, pay);
+        if (ln.op == DIL_OP_KEYGEN) std::memcpy(ln.h_in + L.payload + i * pay, r->seed, pay);
+        if (ln.op != DIL_OP_KEYGEN) {
+            if (r->mlen) std::memcpy(ln.h_in + L.msgs + pos, r->msg, r->mlen);
+            pos += al8(r->mlen);
         }
-        if (e) return e;
-        out += al8(ng * out_item_bytes(op, mode));
-        if ((e = (int)hipEventRecord(ln.join[g], gs))) return e;
-        if ((e = (int)hipStreamWaitEvent(ln.st, ln.join[g], 0))) return e;
     }
-    if ((e = (int)hipMemcpyAsync(ln.h_out, ln.d_out, out, hipMemcpyDeviceToHost, ln.st))) return e;
+    int e = (int)hipEventRecord(ln.ev0, ln.st);
+    if (e) return e;
+    e = (int)hipMemcpyAsync(ln.d_in, ln.h_in, L.msgs + std::max<size_t>(pos, 8), hipMemcpyHostToDevice, ln.st);
+    if (e) return e;
+    const uint32_t *d_idx = reinterpret_cast<const uint32_t *>(ln.d_in + L.idx);
+    const uint64_t *d_moff = reinterpret_cast<const uint64_t *>(ln.d_in + L.moff);
+    const uint32_t *d_mlen = reinterpret_cast<const uint32_t *>(ln.d_in + L.mlen);
+    const uint8_t *d_msg = ln.d_in + L.msgs, *d_pay = ln.d_in + L.payload;
+    if (ln.op == DIL_OP_SIGN) {
+        // small batches: fewer slots per workgroup.  Without a backlog the
+        // batch is spread over all workgroups, so that each item gets
+        // spb / claim nonces per round and finishes in fewer rounds (latency);
+        // with one, every slot is a distinct item's attempt (throughput).
+        int spb = n >= 8192 ? 32 : (n >= 2048 ? 16 : 8);
+        if ((size_t)spb > cfg.max_batch) spb = 1;
+        if ((e = dil_sign_ctx_set_tiling(ln.ctx, spb, 0, 0))) return e;
+        const size_t blocks = cfg.max_batch / (size_t)spb;
+        const int claim = ln.backlog ? spb
+                                     : (int)std::min<size_t>((size_t)spb,
+                                                             std::max<size_t>(1, (n + blocks - 1) / blocks));
+        if ((e = dil_sign_ctx_set_claim(ln.ctx, claim))) return e;
+        e = dil_sign_device_resident(ln.ctx, ks[ms], d_idx, ln.d_out, sigb, d_msg, d_moff, d_mlen, n,
+                                     DIL_ENGINE_PERSISTENT, ln.st);
+    } else if (ln.op == DIL_OP_VERIFY) {
+        e = dil_verify_device_resident(ks[ms], d_idx, reinterpret_cast<int32_t *>(ln.d_out), d_pay, sigb, d_msg,
+                                       d_moff, d_mlen, n, ln.d_work, ln.work_pitch, ln.st);
+    } else {
+        e = dil_keypair_device(mode, ln.d_out, pkb + skb, ln.d_out + pkb, pkb + skb, d_pay, 32, 0, n, ln.d_work,
+                               ln.work_pitch, ln.st);
+    }
+    if (e) return e;
+    if ((e = (int)hipMemcpyAsync(ln.h_out, ln.d_out, n * out_item_bytes(ln.op, mode), hipMemcpyDeviceToHost, ln.st)))
+        return e;
     return (int)hipEventRecord(ln.ev, ln.st);
 }
 
 void dil_server::finish(Lane &ln, int err) {
-    const size_t n = ln.reqs.size();
-    for (int g = 0; g < 9 && !err; ++g) {
-        const int op = g / 3, mode = MODES[g % 3];
-        const size_t sz = out_item_bytes(op, mode);
-        for (size_t i = ln.g0[g]; i < ln.g0[g + 1]; ++i) {
-            dil_request *r = ln.reqs[i];
-            const uint8_t *src = ln.h_out + ln.out_off[g] + (i - ln.g0[g]) * sz;
-            if (op == DIL_OP_VERIFY)
-                r->verdict = *reinterpret_cast<const int32_t *>(src);
+    const uint64_t t0 = mono_ns();
+    const size_t n = ln.reqs.size(), sz = out_item_bytes(ln.op, ln.mode);
+    float ms = 0.f;
+    if (!err && hipEventElapsedTime(&ms, ln.ev0, ln.ev) == hipSuccess) device_ns[ln.op] += (uint64_t)(ms * 1e6f);
+    for (size_t i = 0; i < n; ++i) {
+        dil_request *r = ln.reqs[i];
+        if (!err) {
+            if (ln.op == DIL_OP_VERIFY)
+                r->verdict = reinterpret_cast<const int32_t *>(ln.h_out)[i];
             else
-                std::memcpy(r->out, src, sz);
+                std::memcpy(r->out, ln.h_out + i * sz, sz);
         }
+        publish(r, err);
     }
-    for (size_t i = 0; i < n; ++i) publish(ln.reqs[i], err);
     ln.reqs.clear();
-    ln.busy = false;
+    finish_ns[ln.op] += mono_ns() - t0;
     completed.fetch_add(n, std::memory_order_acq_rel);
     std::lock_guard<std::mutex> g(done_mu);
     cv_done.notify_all();
 }
 
-void dil_server::run() {
+void dil_server::lane_loop(Lane &ln) {
     (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-        bool progress = false;
-        // harvest finished lanes (without the queue lock)
-        lk.unlock();
-        for (auto &ln : lanes)
-            if (ln.busy) {
-                const hipError_t q = hipEventQuery(ln.ev);
-                if (q == hipSuccess) {
-                    finish(ln, 0);
-                    progress = true;
-                } else if (q != hipErrorNotReady) {
-                    finish(ln, (int)q);
-                    progress = true;
-                }
-            }
-        lk.lock();
-        // launch onto free lanes
-        for (auto &ln : lanes) {
-            if (ln.busy || npending == 0) continue;
-            if (!fill(ln)) break;
-            ln.busy = true;
-            lk.unlock();
-            const int e = enqueue(ln);
-            if (e) finish(ln, e);
-            lk.lock();
-            progress = true;
+        auto &dq = q[ln.group][ln.cls];
+        if (dq.empty()) {
+            if (stop) break;
+            ln.cv->wait(lk);
+            continue;
         }
-        bool any_busy = false;
-        for (auto &ln : lanes) any_busy |= ln.busy;
-        if (stop && npending == 0 && !any_busy) break;
-        if (progress) continue;
-        if (any_busy || npending)
-            cv_sub.wait_for(lk, std::chrono::microseconds(any_busy ? 20 : std::max<uint32_t>(cfg.batch_window_us, 1) / 4 + 1));
-        else
-            cv_sub.wait(lk, [&] { return npending > 0 || stop; });
+        if (!fill(ln)) {  // holding a partial batch for the window
+            ln.cv->wait_for(lk, std::chrono::microseconds(std::max<uint32_t>(cfg.batch_window_us, 4) / 4));
+            continue;
+        }
+        lk.unlock();
+        const uint64_t t0 = mono_ns();
+        int e = enqueue(ln);
+        stage_ns[ln.op] += mono_ns() - t0;
+        ++nb_op[ln.op];
+        // the completion event is a blocking-sync event: the thread sleeps in
+        // the driver instead of polling the runtime (15 polling threads
+        // contend with each other's launches)
+        if (!e) e = (int)hipEventSynchronize(ln.ev);
+        finish(ln, e);
+        lk.lock();
     }
 }
 
@@ -342,47 +323,71 @@ extern "C" {
 uint64_t dil_monotonic_ns(void) { return mono_ns(); }
 
 int dil_server_create(dil_server **out, const dil_server_config *cfg) {
-    if (!out || !cfg || cfg->lanes == 0 || cfg->max_batch == 0 || cfg->max_batch > 65536) return DIL_EINVAL;
+    if (!out || !cfg || cfg->lanes == 0 || cfg->lanes > 16 || cfg->max_batch == 0 || cfg->max_batch > 65536)
+        return DIL_EINVAL;
     dil_server *s = new dil_server();
     s->cfg = *cfg;
     (void)hipGetDevice(&s->device);
     const size_t b = cfg->max_batch;
-    // per lane: outputs, fixed inputs, work rows, signing contexts, messages
-    s->out_cap = b * std::max(MAX_SIG, MAX_KEYPAIR) + 9 * 8;
-    s->in_fixed = InLayout(b, b * MAX_SIG).msgs;
-    size_t wrow = 0;
-    for (int m : MODES)
-        wrow = std::max({wrow, dil_verify_work_row_bytes(m), al256(dil_keypair_work_row_bytes(m))});
-    s->work_pitch = al256(wrow);
-    size_t ctx_bytes = 0;
-    for (int m : MODES) ctx_bytes += sign_ctx_bytes(m, b);
-    const size_t fixed = s->out_cap + s->in_fixed + b * s->work_pitch + ctx_bytes;
-    const size_t per_lane = cfg->hbm_budget_bytes / cfg->lanes;
+    // every lane's fixed part (index arrays, payload, outputs, work rows,
+    // signing context) first; the rest of the budget is split evenly into
+    // the message areas of the sign and verify lanes
+    s->long_bytes = cfg->long_message_bytes;
+    std::vector<std::pair<int, int>> kinds;  // (group, class) of each lane
+    for (uint32_t k = 0; k < cfg->lanes; ++k)
+        for (int g = 0; g < NGROUPS; ++g) {
+            kinds.push_back({g, 0});
+            if (s->long_bytes && g >= 3) kinds.push_back({g, 1});
+        }
+    s->lanes.resize(kinds.size());
+    size_t fixed = 0, msg_lanes = 0, short_msg = 0;
+    for (size_t i = 0; i < s->lanes.size(); ++i) {
+        Lane &ln = s->lanes[i];
+        ln.group = kinds[i].first;
+        ln.cls = kinds[i].second;
+        ln.op = ln.group / 3;
+        ln.mode = MODES[ln.group % 3];
+        ln.in_fixed = InLayout(b, b * payload_item_bytes(ln.op, ln.mode)).msgs;
+        if (ln.op != DIL_OP_SIGN) {
+            const size_t w = ln.op == DIL_OP_VERIFY ? dil_verify_work_row_bytes(ln.mode)
+                                                    : dil_keypair_work_row_bytes(ln.mode);
+            ln.work_pitch = al256(w);
+        }
+        fixed += ln.in_fixed + 8 + b * out_item_bytes(ln.op, ln.mode) + b * ln.work_pitch +
+                 (ln.op == DIL_OP_SIGN ? sign_ctx_bytes(ln.mode, b) : 0);
+        // short-message lanes of a split server hold at most b messages of
+        // long_message_bytes; the long-message (or unsplit) lanes share the rest
+        if (ln.op != DIL_OP_KEYGEN) {
+            if (s->long_bytes && ln.cls == 0)
+                short_msg += b * al8(s->long_bytes);
+            else
+                ++msg_lanes;
+        }
+    }
     int e = 0;
-    if (per_lane <= fixed + 4096) e = DIL_ENOMEM;
-    s->msg_cap = e ? 0 : per_lane - fixed;
-    s->lane_bytes = per_lane;
+    const size_t budget = cfg->hbm_budget_bytes;
+    if (budget <= fixed + short_msg + msg_lanes * 4096) e = DIL_ENOMEM;
+    s->msg_cap = e ? 0 : ((budget - fixed - short_msg) / msg_lanes) & ~size_t(255);
     for (int i = 0; i < 3 && !e; ++i)
         if (cfg->max_keys[i]) e = dil_keystore_create(&s->ks[i], MODES[i], cfg->max_keys[i]);
-    s->lanes.resize(cfg->lanes);
     for (auto &ln : s->lanes) {
         if (e) break;
         auto chk = [&](int r) {
             if (r && !e) e = r;
         };
-        chk((int)hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking));
-        chk((int)hipEventCreateWithFlags(&ln.ev, hipEventDisableTiming));
-        chk((int)hipEventCreateWithFlags(&ln.fork, hipEventDisableTiming));
-        for (int g = 0; g < 9; ++g) {
-            chk((int)hipStreamCreateWithFlags(&ln.sub[g], hipStreamNonBlocking));
-            chk((int)hipEventCreateWithFlags(&ln.join[g], hipEventDisableTiming));
-        }
-        chk((int)hipHostMalloc((void **)&ln.h_in, s->in_fixed + s->msg_cap, hipHostMallocDefault));
-        chk((int)hipHostMalloc((void **)&ln.h_out, s->out_cap, hipHostMallocDefault));
-        chk((int)hipMalloc((void **)&ln.d_in, s->in_fixed + s->msg_cap));
-        chk((int)hipMalloc((void **)&ln.d_out, s->out_cap));
-        chk((int)hipMalloc((void **)&ln.d_work, b * s->work_pitch));
-        for (int i = 0; i < 3 && !e; ++i) chk(dil_sign_ctx_create_resident(&ln.ctx[i], MODES[i], b, b));
+        ln.msg_cap = ln.op == DIL_OP_KEYGEN ? 0 : (s->long_bytes && ln.cls == 0 ? b * al8(s->long_bytes) : s->msg_cap);
+        const size_t in_bytes = ln.in_fixed + ln.msg_cap + 8, out_bytes = b * out_item_bytes(ln.op, ln.mode);
+        s->lane_bytes_max = std::max(s->lane_bytes_max, in_bytes + out_bytes + b * ln.work_pitch +
+                                                            (ln.op == DIL_OP_SIGN ? sign_ctx_bytes(ln.mode, b) : 0));
+        chk(create_lane_stream(&ln.st));
+        chk((int)hipEventCreateWithFlags(&ln.ev, hipEventBlockingSync));
+        chk((int)hipEventCreate(&ln.ev0));
+        chk((int)hipHostMalloc((void **)&ln.h_in, in_bytes, hipHostMallocDefault));
+        chk((int)hipHostMalloc((void **)&ln.h_out, out_bytes, hipHostMallocDefault));
+        chk((int)hipMalloc((void **)&ln.d_in, in_bytes));
+        chk((int)hipMalloc((void **)&ln.d_out, out_bytes));
+        if (ln.work_pitch) chk((int)hipMalloc((void **)&ln.d_work, b * ln.work_pitch));
+        if (ln.op == DIL_OP_SIGN) chk(dil_sign_ctx_create_resident(&ln.ctx, ln.mode, b, b));
     }
     if (e) {
         for (auto &ln : s->lanes) free_lane(ln);
@@ -390,7 +395,12 @@ int dil_server_create(dil_server **out, const dil_server_config *cfg) {
         delete s;
         return e;
     }
-    s->worker = std::thread([s] { s->run(); });
+    for (size_t i = 0; i < s->lanes.size(); ++i) {
+        s->cvs.emplace_back(new std::condition_variable());
+        s->lanes[i].cv = s->cvs.back().get();
+        s->lanes_of[s->lanes[i].group][s->lanes[i].cls].push_back((int)i);
+    }
+    for (auto &ln : s->lanes) s->workers.emplace_back([s, &ln] { s->lane_loop(ln); });
     *out = s;
     return 0;
 }
@@ -401,8 +411,8 @@ int dil_server_destroy(dil_server *s) {
         std::lock_guard<std::mutex> g(s->mu);
         s->stop = true;
     }
-    s->cv_sub.notify_all();
-    s->worker.join();
+    for (auto &cv : s->cvs) cv->notify_all();
+    for (auto &t : s->workers) t.join();
     for (auto &ln : s->lanes) free_lane(ln);
     for (auto *k : s->ks) dil_keystore_destroy(k);
     delete s;
@@ -418,6 +428,7 @@ int dil_server_submit(dil_server *s, dil_request *const *reqs, size_t n) {
     if (!s || (n && !reqs)) return DIL_EINVAL;
     const uint64_t t = mono_ns();
     size_t queued = 0, rejected = 0;
+    uint32_t touched = 0;  // bit 2 * group + class of every queue that got work
     {
         std::lock_guard<std::mutex> g(s->mu);
         for (size_t i = 0; i < n; ++i) {
@@ -442,7 +453,9 @@ int dil_server_submit(dil_server *s, dil_request *const *reqs, size_t n) {
                 s->publish(r, bad);
                 continue;
             }
-            s->q[r->op][ms].push_back(r);
+            const int cls = (s->long_bytes && r->op != DIL_OP_KEYGEN && r->mlen > s->long_bytes) ? 1 : 0;
+            s->q[r->op * 3 + ms][cls].push_back(r);
+            touched |= 1u << (2 * (r->op * 3 + ms) + cls);
             ++queued;
         }
         s->npending += queued;
@@ -452,7 +465,9 @@ int dil_server_submit(dil_server *s, dil_request *const *reqs, size_t n) {
         std::lock_guard<std::mutex> g(s->done_mu);
         s->cv_done.notify_all();
     }
-    if (queued) s->cv_sub.notify_one();
+    for (int k = 0; k < 2 * NGROUPS; ++k)
+        if (touched & (1u << k))
+            for (int li : s->lanes_of[k / 2][k % 2]) s->lanes[li].cv->notify_one();
     return 0;
 }
 
@@ -491,8 +506,14 @@ int dil_server_get_stats(dil_server *s, dil_server_stats *st) {
     st->batches = s->batches;
     st->rejected = s->rejected;
     for (int i = 0; i < 3; ++i) st->items_by_op[i] = s->by_op[i];
-    st->lane_bytes = s->lane_bytes;
+    st->lane_bytes = s->lane_bytes_max;
     st->max_msg_bytes = s->msg_cap;
+    for (int i = 0; i < 3; ++i) {
+        st->batches_by_op[i] = s->nb_op[i];
+        st->stage_ns[i] = s->stage_ns[i];
+        st->device_ns[i] = s->device_ns[i];
+        st->finish_ns[i] = s->finish_ns[i];
+    }
     return 0;
 }
 

@@ -19,8 +19,13 @@ namespace dil {
 
 constexpr int SIGN_WAVES = 4;
 
+// One lane absorbs a whole message serially, so the batch waits for its
+// longest message; the wave asks for the highest issue priority on its SIMD
+// (s_setprio, a per-wave hint) so that other kernels' waves sharing the SIMD
+// do not stretch that serial chain.
 template <class P>
 __global__ __launch_bounds__(256) void sg_mu_kernel(SignArgs a) {
+    __builtin_amdgcn_s_setprio(3);
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.n) stage_mu<P>(a, i);
 }

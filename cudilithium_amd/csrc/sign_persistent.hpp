@@ -51,7 +51,8 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
     __shared__ int own_done[S];
     __shared__ uint32_t slot_own[S], slot_nonce[S];
     __shared__ int slot_ok[S];
-    __shared__ int n_own, queue_empty;
+    __shared__ uint8_t w_unit[S];  // w work units: slot s, paired with s + u when s / u is even
+    __shared__ int n_own, queue_empty, n_units;
     const int tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
     constexpr int NW = PERSIST_THREADS / 64;
     const size_t slot0 = (size_t)blockIdx.x * S;
@@ -104,6 +105,12 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
                 own_next[o] += (S - o + u - 1) / u;
                 own_done[o] = 0;
             }
+            // slots s and s + u carry consecutive nonces of the same item: the
+            // w stage takes them as one unit (one pass over that item's A)
+            int nu = 0;
+            for (int s = 0; s < S && u > 0; ++s)
+                if ((s / u) % 2 == 0) w_unit[nu++] = (uint8_t)s;
+            n_units = nu;
         }
         __syncthreads();
         mark(0);
@@ -117,7 +124,21 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
         }
         __syncthreads();
         mark(1);
-        for (int s = wave; s < S; s += NW) stage_w<P>(a, own_item[slot_own[s]], slot_base(s), x[wave], zt);
+        for (int t = wave; t < n_units; t += NW) {
+            const int s = w_unit[t];
+            const size_t item = own_item[slot_own[s]];
+            if constexpr (P::L <= 5) {
+                if (s + u < S) {
+                    uint8_t *const pair[2] = {slot_base(s), slot_base(s + u)};
+                    stage_w_multi<P, 2>(a, item, pair, x[wave], zt);
+                    continue;
+                }
+            } else if (s + u < S) {
+                // L = 7: two y vectors and two A rows exceed the register budget
+                stage_w<P>(a, item, slot_base(s + u), x[wave], zt);
+            }
+            stage_w<P>(a, item, slot_base(s), x[wave], zt);
+        }
         __syncthreads();
         mark(2);
         for (int s = tid; s < S; s += PERSIST_THREADS) stage_c<P>(a, own_item[slot_own[s]], slot_base(s), slot_sig(s));

@@ -179,25 +179,30 @@ __device__ __forceinline__ void load_y_A(int32_t (&v)[4], const uint8_t *slot, i
     for (int r = 0; r < 4; ++r) v[r] = P::GAMMA1 - (int32_t)f[r];
 }
 
-template <class P>
-__device__ __forceinline__ void stage_w(const SignArgs &a, size_t item, uint8_t *slot, int32_t *x, const ZetaLds &zt) {
+// w for M attempts of the same item (slots[0..M)) in one wave: every A row
+// is loaded once for all M products, and the M transforms give the wave
+// independent work to overlap with the loads.
+template <class P, int M>
+__device__ __forceinline__ void stage_w_multi(const SignArgs &a, size_t item, uint8_t *const (&slots)[M], int32_t *x,
+                                              const ZetaLds &zt) {
     using Z = Sizes<P>;
     using SL = SlotLayout<P>;
     const ItemView<P> it(a, item);
     // A rows are streamed from HBM with the next row's loads issued before
-    // the current row's products and INTT (software prefetch: the wave has
-    // an INTT's worth of work to cover each row's load latency)
+    // the current row's products and INTTs (software prefetch)
     int32_t an[P::L][4];
 #pragma unroll
     for (int l = 0; l < P::L; ++l) wave_load_D(an[l], it.mat + l * N);
-    int32_t yh[P::L][4];
+    int32_t yh[M][P::L][4];
 #pragma unroll
-    for (int l = 0; l < P::L; ++l) {
-        load_y_A<P>(yh[l], slot, l);
-        ntt_wave(yh[l], x, zt);
-    }
+    for (int j = 0; j < M; ++j)
+#pragma unroll
+        for (int l = 0; l < P::L; ++l) {
+            load_y_A<P>(yh[j][l], slots[j], l);
+            ntt_wave(yh[j][l], x, zt);
+        }
     for (int k = 0; k < P::K; ++k) {
-        int32_t acc[4] = {0, 0, 0, 0};
+        int32_t acc[M][4];
         int32_t ac[P::L][4];
 #pragma unroll
         for (int l = 0; l < P::L; ++l)
@@ -208,27 +213,40 @@ __device__ __forceinline__ void stage_w(const SignArgs &a, size_t item, uint8_t 
             for (int l = 0; l < P::L; ++l) wave_load_D(an[l], it.mat + ((k + 1) * P::L + l) * N);
         }
 #pragma unroll
-        for (int l = 0; l < P::L; ++l) {
+        for (int j = 0; j < M; ++j) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc[r] += mont_mul(ac[l][r], yh[l][r]);
+            for (int r = 0; r < 4; ++r) acc[j][r] = 0;
+#pragma unroll
+            for (int l = 0; l < P::L; ++l)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[j][r] += mont_mul(ac[l][r], yh[j][l][r]);
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = reduce32(acc[r]);
-        intt_wave(acc, x, zt);
-        int32_t w1[4];
+        for (int j = 0; j < M; ++j) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            acc[r] = caddq(acc[r]);
-            int32_t w0;
-            w1[r] = decompose<P>(acc[r], w0);
+            for (int r = 0; r < 4; ++r) acc[j][r] = reduce32(acc[j][r]);
+            intt_wave(acc[j], x, zt);
+            int32_t w1[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[j][r] = caddq(acc[j][r]);
+                int32_t w0;
+                w1[r] = decompose<P>(acc[j][r], w0);
+            }
+            wave_store_A(reinterpret_cast<int32_t *>(slots[j] + SL::W) + k * N, acc[j]);
+            layout_A_to_D(w1, x);
+            uint32_t u[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[r] = (uint32_t)w1[r];
+            wave_pack_D<Z::W1_BITS>(slots[j] + SL::W1PK + k * Z::POLYW1_BYTES, u);
         }
-        wave_store_A(reinterpret_cast<int32_t *>(slot + SL::W) + k * N, acc);
-        layout_A_to_D(w1, x);
-        uint32_t u[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) u[r] = (uint32_t)w1[r];
-        wave_pack_D<Z::W1_BITS>(slot + SL::W1PK + k * Z::POLYW1_BYTES, u);
     }
+}
+
+template <class P>
+__device__ __forceinline__ void stage_w(const SignArgs &a, size_t item, uint8_t *slot, int32_t *x, const ZetaLds &zt) {
+    uint8_t *const slots[1] = {slot};
+    stage_w_multi<P, 1>(a, item, slots, x, zt);
 }
 
 template <class P>

@@ -25,6 +25,8 @@ template <class P>
 __global__ __launch_bounds__(256) void vf_prep_kernel(VerifyArgs a) {
     using Z = Sizes<P>;
     using VL = VerifyLayout<P>;
+    // serial per-lane message absorb: highest wave issue priority (see sg_mu_kernel)
+    __builtin_amdgcn_s_setprio(3);
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const uint8_t *sig = a.sig + i * a.sig_pitch;
@@ -258,9 +260,11 @@ template <class P, bool RES>
 static int launch_vf_main(const VerifyArgs &a, hipStream_t st) {
     constexpr int G = VfMainCfg<P>::G;
     const size_t smem = sizeof(VfShared<P, RES>);
-    hipError_t e = hipFuncSetAttribute((const void *)vf_main_kernel<P, RES>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (e != hipSuccess) return (int)e;
+    // the attribute is set once per kernel (a host call that need not be
+    // repeated per launch; it is not a stream operation)
+    static const hipError_t attr = hipFuncSetAttribute((const void *)vf_main_kernel<P, RES>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (attr != hipSuccess) return (int)attr;
     hipLaunchKernelGGL((vf_main_kernel<P, RES>), dim3((unsigned)((a.n + G - 1) / G)), dim3(256), smem, st, a);
     return 0;
 }

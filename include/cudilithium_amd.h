@@ -316,9 +316,13 @@ int dil_graph_destroy(dil_graph *graph);
 /* ---------------------------------------------------------------------- */
 /* Serving loop (no counterpart in the reference, whose callers batch by   */
 /* hand, benches/bench_dilithium.cu:186-239).  Single requests from any    */
-/* host thread are batched continuously per (op, mode) onto `lanes`        */
-/* in-flight batches, each with device buffers preallocated from a fixed   */
-/* HBM budget; sign and verify use resident keys.                         */
+/* host thread are batched continuously per (op, mode) group, each group   */
+/* with `lanes` batches in flight on streams of its own and device buffers */
+/* preallocated from a fixed HBM budget; sign and verify use resident      */
+/* keys.  The server runs (9 + 6 with a long-message split) * lanes        */
+/* streams: give the process about twice that many hardware queues         */
+/* (GPU_MAX_HW_QUEUES, HIP's default is 4; 32 for the default server), or  */
+/* lanes share queues and serialise.                                      */
 /* ---------------------------------------------------------------------- */
 
 #define DIL_OP_KEYGEN 0
@@ -327,10 +331,12 @@ int dil_graph_destroy(dil_graph *graph);
 
 typedef struct dil_server_config {
   size_t hbm_budget_bytes;  /* device bytes for in-flight batches (all lanes) */
-  uint32_t lanes;           /* batches in flight at once (>= 1) */
+  uint32_t lanes;           /* batches in flight per (op, mode) group (1..16) */
   uint32_t max_batch;       /* requests per batch (<= 65536) */
   uint32_t max_keys[3];     /* resident key capacity of modes 2, 3, 5 */
   uint32_t batch_window_us; /* hold a partial batch this long for more requests */
+  uint32_t long_message_bytes; /* sign / verify messages above this (bytes) get
+                                  lanes of their own (0: no split) */
 } dil_server_config;
 
 /* One request.  The caller owns it and every buffer it points to until
@@ -357,8 +363,13 @@ typedef struct dil_request {
 typedef struct dil_server_stats {
   uint64_t submitted, completed, batches, rejected;
   uint64_t items_by_op[3];
-  uint64_t lane_bytes;      /* device bytes per lane */
-  uint64_t max_msg_bytes;   /* largest total message bytes one batch admits */
+  uint64_t lane_bytes;      /* device bytes of the largest lane */
+  uint64_t max_msg_bytes;   /* total message bytes one sign / verify batch admits */
+  /* per op (keygen, sign, verify), summed over its batches: */
+  uint64_t batches_by_op[3];
+  uint64_t stage_ns[3];     /* host: gather inputs into the pinned buffer, enqueue */
+  uint64_t device_ns[3];    /* stream time from input copy to output copy, HIP events */
+  uint64_t finish_ns[3];    /* host: scatter outputs into the requests */
 } dil_server_stats;
 
 typedef struct dil_server dil_server;
