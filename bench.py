@@ -5,10 +5,12 @@ Headline workload (default): Dilithium3 deterministic signing of 16384
 seeds, inputs resident in HBM, one fixed-stage pass plus one persistent
 rejection-loop launch per step.  A step signs the whole batch.
 
-Printed on one JSON line: throughput (signatures/s), the roofline of the
-dominant kernel (the persistent rejection loop, integer-VALU bound), the
-oracle's CPU rate on a bounded sample, and the per-stage (reference
-architecture) GPU form on the same inputs.
+Printed on one JSON line: throughput (signatures/s, the persistent engine
+replayed as a HIP graph), the roofline of the dominant kernel (the persistent
+rejection loop, integer-VALU bound), the oracle's CPU rate on a bounded
+sample, the per-stage (reference architecture) GPU form on the same inputs,
+and a serving summary (resident keys, continuous batching: p50 / p99 latency
+at two offered loads).
 
     python bench.py [--steps K] [--warmup W] [--workload sign3|sign2|sign5|verify5|ntt|copy|serve]
 """
@@ -24,15 +26,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-# The serving loop runs each (op, mode, message class) lane on its own stream
-# (15 streams); HIP maps streams onto at most GPU_MAX_HW_QUEUES hardware
-# queues (default 4) and streams that share a queue serialise -- a short
-# message's batch then waits behind a long message's hash.  Measured: with 16
-# queues lanes still shared queues, with 24-32 they did not.  Read by the HIP
-# runtime at initialisation, so it is set before anything loads it (an
-# explicit setting in the environment wins).
-if "--workload" in sys.argv and "serve" in sys.argv:
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "32")
+
 
 METRIC = "signs/s & verifies/s per MI355X (Dil2/3/5), bit-exact; serving p50/p99 latency"
 PEAK_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # CUs x SIMDs x lanes x GHz = 39.3 T int32 ops/s
@@ -42,7 +36,7 @@ PEAK_HBM_GBS = 8000.0
 # WRITE_SIZE passes scaled by a 1 GiB copy's readings (read = 2 x FETCH_SIZE,
 # written = WRITE_SIZE); profiles/sign3_16384_hbm_valu_counters.md.
 TRAFFIC_BYTES = {
-    "dilithium3_sign_16384_msg_loguniform_32B-4KiB": (2 * 2241688 + 1317875) * 1024,
+    "dilithium3_sign_16384_msg_loguniform_32B-4KiB": (2 * 1760315 + 1334274) * 1024,
 }
 
 
@@ -466,6 +460,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perstage", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="sign3: skip the serving summary")
     ap.add_argument("--requests", type=int, default=40000, help="serve: requests per run")
     ap.add_argument("--budget-gib", type=float, default=4.0, help="serve: HBM budget of the in-flight batches")
     ap.add_argument("--lanes", type=int, default=1, help="serve: batches in flight per (op, mode) group")
@@ -486,6 +481,20 @@ def main():
     w = args.workload
     if w.startswith("sign"):
         res = bench_sign(args, int(w[4:] or 3))
+        if w == "sign3" and not args.no_serve:
+            # the serving mix beside the headline: 10,000 requests at two fixed
+            # offered loads (the full sweep is --workload serve)
+            import copy
+            sargs = copy.copy(args)
+            sargs.requests, sargs.fractions, sargs.rates = 10000, [], [25000.0, 50000.0]
+            srv = bench_serve(sargs)
+            res["serving"] = {"workload": srv["config"]["workload"], "parity": srv["parity"],
+                              "saturation_rps": srv["value"], "hbm_budget_bytes": srv["config"]["hbm_budget_bytes"],
+                              "loads": [{k: r[k] for k in ("offered_rps", "throughput_rps", "p50_us", "p99_us",
+                                                           "p50_us_short", "p99_us_short", "p50_us_long",
+                                                           "p99_us_long", "long_fraction")} for r in srv["loads"]]}
+            if srv["parity"] != "bit-exact":
+                res["parity"] = "MISMATCH"
     elif w.startswith("verify"):
         res = bench_verify(args, int(w[6:] or 5))
     elif w == "ntt":

@@ -26,10 +26,9 @@
 // memory -- every lane's buffers and signing context and the message areas
 // -- is carved from the budget at creation, so admission never allocates; a
 // message larger than its group's message area is rejected (DIL_ERANGE).
-// Streams that the HIP runtime maps onto one hardware queue serialise; it
-// has at most GPU_MAX_HW_QUEUES of them (default 4), and with as many queues
-// as streams lanes were still seen sharing queues, so a server with 15 lanes
-// wants 24-32.
+// Every lane's stream gets a hardware queue of its own (create_lane_stream):
+// streams that share a queue serialise, and measured with plain streams a
+// short message's batch waited behind a long message's hash.
 #include <hip/hip_runtime.h>
 #include <time.h>
 

@@ -319,10 +319,8 @@ int dil_graph_destroy(dil_graph *graph);
 /* host thread are batched continuously per (op, mode) group, each group   */
 /* with `lanes` batches in flight on streams of its own and device buffers */
 /* preallocated from a fixed HBM budget; sign and verify use resident      */
-/* keys.  The server runs (9 + 6 with a long-message split) * lanes        */
-/* streams: give the process about twice that many hardware queues         */
-/* (GPU_MAX_HW_QUEUES, HIP's default is 4; 32 for the default server), or  */
-/* lanes share queues and serialise.                                      */
+/* keys.  Each lane's stream has a hardware queue of its own (a stream     */
+/* with an all-CU mask), so lanes never serialise behind each other.       */
 /* ---------------------------------------------------------------------- */
 
 #define DIL_OP_KEYGEN 0
