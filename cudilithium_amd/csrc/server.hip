@@ -28,7 +28,13 @@
 // message larger than its group's message area is rejected (DIL_ERANGE).
 // Every lane's stream gets a hardware queue of its own (create_lane_stream):
 // streams that share a queue serialise, and measured with plain streams a
-// short message's batch waited behind a long message's hash.
+// short message's batch waited behind a long message's hash.  For the same
+// reason the staging copies are kernels on the lane's stream that read and
+// write the pinned buffers directly (stage_copy_kernel): the runtime sends
+// larger host<->device copies to DMA engines shared by every stream, where a
+// short batch's copy queued behind a long batch's result copy, which waits
+// for that batch's hash (gaps of ~15 ms on the short lanes in a kernel
+// trace of the serving mix).
 #include <hip/hip_runtime.h>
 #include <time.h>
 
@@ -101,7 +107,8 @@ struct Lane {
     int group = 0, op = 0, mode = 0, cls = 0;  // cls 1: messages above long_message_bytes
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr, ev0 = nullptr;  // end (completion), start (timing)
-    uint8_t *h_in = nullptr, *h_out = nullptr;
+    uint8_t *h_in = nullptr, *h_out = nullptr;        // pinned staging (host view)
+    uint8_t *h_in_dev = nullptr, *h_out_dev = nullptr;  // the same buffers, device view
     uint8_t *d_in = nullptr, *d_out = nullptr, *d_work = nullptr;
     size_t in_fixed = 0, msg_cap = 0, work_pitch = 0;
     dil_sign_ctx *ctx = nullptr;  // sign lanes
@@ -124,6 +131,22 @@ int create_lane_stream(hipStream_t *st) {
     std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0xFFFFFFFFu);
     if (cus % 32) mask.back() = (1u << (cus % 32)) - 1u;
     return (int)hipExtStreamCreateWithCUMask(st, (uint32_t)mask.size(), mask.data());
+}
+
+// Copies n16 16-byte words (both pointers 16-byte aligned); one side is a
+// pinned host buffer's device view.  Grid-stride, so any grid size covers n16.
+__global__ __launch_bounds__(256) void stage_copy_kernel(uint4 *dst, const uint4 *src, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int stage_copy(uint8_t *dst, const uint8_t *src, size_t bytes, hipStream_t st) {
+    const size_t n16 = (bytes + 15) / 16;
+    if (n16 == 0) return 0;
+    const unsigned blocks = (unsigned)std::min<size_t>((n16 + 255) / 256, 1024);
+    hipLaunchKernelGGL(stage_copy_kernel, dim3(blocks), dim3(256), 0, st, reinterpret_cast<uint4 *>(dst),
+                       reinterpret_cast<const uint4 *>(src), n16);
+    return (int)hipGetLastError();
 }
 
 void free_lane(Lane &ln) {
@@ -233,8 +256,8 @@ int dil_server::enqueue(Lane &ln) {
     }
     int e = (int)hipEventRecord(ln.ev0, ln.st);
     if (e) return e;
-    e = (int)hipMemcpyAsync(ln.d_in, ln.h_in, L.msgs + std::max<size_t>(pos, 8), hipMemcpyHostToDevice, ln.st);
-    if (e) return e;
+    // (L.msgs + pos <= in_fixed + msg_cap; the buffers hold that rounded up to 16 bytes)
+    if ((e = stage_copy(ln.d_in, ln.h_in_dev, L.msgs + std::max<size_t>(pos, 8), ln.st))) return e;
     const uint32_t *d_idx = reinterpret_cast<const uint32_t *>(ln.d_in + L.idx);
     const uint64_t *d_moff = reinterpret_cast<const uint64_t *>(ln.d_in + L.moff);
     const uint32_t *d_mlen = reinterpret_cast<const uint32_t *>(ln.d_in + L.mlen);
@@ -262,8 +285,7 @@ int dil_server::enqueue(Lane &ln) {
                                ln.work_pitch, ln.st);
     }
     if (e) return e;
-    if ((e = (int)hipMemcpyAsync(ln.h_out, ln.d_out, n * out_item_bytes(ln.op, mode), hipMemcpyDeviceToHost, ln.st)))
-        return e;
+    if ((e = stage_copy(ln.h_out_dev, ln.d_out, n * out_item_bytes(ln.op, mode), ln.st))) return e;
     return (int)hipEventRecord(ln.ev, ln.st);
 }
 
@@ -352,7 +374,8 @@ int dil_server_create(dil_server **out, const dil_server_config *cfg) {
                                                     : dil_keypair_work_row_bytes(ln.mode);
             ln.work_pitch = al256(w);
         }
-        fixed += ln.in_fixed + 8 + b * out_item_bytes(ln.op, ln.mode) + b * ln.work_pitch +
+        fixed += ln.in_fixed + 8 + b * out_item_bytes(ln.op, ln.mode) + 1024 /* staging rounding */ +
+                 b * ln.work_pitch +
                  (ln.op == DIL_OP_SIGN ? sign_ctx_bytes(ln.mode, b) : 0);
         // short-message lanes of a split server hold at most b messages of
         // long_message_bytes; the long-message (or unsplit) lanes share the rest
@@ -375,14 +398,20 @@ int dil_server_create(dil_server **out, const dil_server_config *cfg) {
             if (r && !e) e = r;
         };
         ln.msg_cap = ln.op == DIL_OP_KEYGEN ? 0 : (s->long_bytes && ln.cls == 0 ? b * al8(s->long_bytes) : s->msg_cap);
-        const size_t in_bytes = ln.in_fixed + ln.msg_cap + 8, out_bytes = b * out_item_bytes(ln.op, ln.mode);
+        // staging sizes rounded up for the 16-byte copy words
+        const size_t in_bytes = al256(ln.in_fixed + ln.msg_cap + 8) + 256,
+                     out_bytes = al256(b * out_item_bytes(ln.op, ln.mode)) + 256;
         s->lane_bytes_max = std::max(s->lane_bytes_max, in_bytes + out_bytes + b * ln.work_pitch +
                                                             (ln.op == DIL_OP_SIGN ? sign_ctx_bytes(ln.mode, b) : 0));
         chk(create_lane_stream(&ln.st));
         chk((int)hipEventCreateWithFlags(&ln.ev, hipEventBlockingSync));
         chk((int)hipEventCreate(&ln.ev0));
-        chk((int)hipHostMalloc((void **)&ln.h_in, in_bytes, hipHostMallocDefault));
-        chk((int)hipHostMalloc((void **)&ln.h_out, out_bytes, hipHostMallocDefault));
+        // coherent: the copy kernels' host writes are visible once the lane's event completes
+        chk((int)hipHostMalloc((void **)&ln.h_in, in_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+        chk((int)hipHostMalloc((void **)&ln.h_out, out_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+        if (!e) chk((int)hipHostGetDevicePointer((void **)&ln.h_in_dev, ln.h_in, 0));
+        if (!e) chk((int)hipHostGetDevicePointer((void **)&ln.h_out_dev, ln.h_out, 0));
+        if (!e && (!ln.h_in_dev || !ln.h_out_dev)) e = DIL_EINVAL;
         chk((int)hipMalloc((void **)&ln.d_in, in_bytes));
         chk((int)hipMalloc((void **)&ln.d_out, out_bytes));
         if (ln.work_pitch) chk((int)hipMalloc((void **)&ln.d_work, b * ln.work_pitch));
