@@ -36,6 +36,9 @@ constexpr uint32_t PERSIST_MAX_ROUNDS = 20000;
 // Optional phase profile: PERSIST_PHASES uint64 cycle sums (assign, y, w, c,
 // z, commit) at a separate pointer, null when not profiling.
 constexpr int PERSIST_PHASES = 6;
+// wave priority (s_setprio) of the Keccak stages; 0 elsewhere.  Measured on
+// 16384-item batches: +2-3 % signatures/s for Dilithium3 over no priority.
+constexpr int KECCAK_PRIO = 2;
 
 // WPE: waves per SIMD the register allocation must allow (512 / WPE VGPRs)
 // claim: at most this many items owned per workgroup (1..S); below S every
@@ -44,7 +47,6 @@ constexpr int PERSIST_PHASES = 6;
 template <class P, int S, int WPE>
 __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void sg_persistent_kernel(
     SignArgs a, SlotRef sr, uint32_t *head, int claim, unsigned long long *phase_sums) {
-    using Z = Sizes<P>;
     __shared__ ZetaLds zt;
     __shared__ int32_t x[PERSIST_THREADS / 64][NTT_XBUF];
     __shared__ uint32_t own_item[S], own_next[S];
@@ -118,10 +120,15 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
         if (u == 0) break;
         if (tid == 0) atomicAdd(head + 2, (uint32_t)S);  // attempts executed (incl. speculative)
 
+        // the Keccak stages (y, c) run few serial permutation chains that the
+        // whole workgroup then waits for; their waves get issue priority over
+        // the transform stages of other workgroups on the same SIMDs
+        __builtin_amdgcn_s_setprio(KECCAK_PRIO);
         for (int t = tid; t < S * P::L; t += PERSIST_THREADS) {
             const int s = t / P::L;
             stage_y<P>(a, own_item[slot_own[s]], slot_nonce[s], t % P::L, slot_base(s));
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         mark(1);
         for (int t = wave; t < n_units; t += NW) {
@@ -141,7 +148,9 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
         }
         __syncthreads();
         mark(2);
+        __builtin_amdgcn_s_setprio(KECCAK_PRIO);
         for (int s = tid; s < S; s += PERSIST_THREADS) stage_c<P>(a, own_item[slot_own[s]], slot_base(s), slot_sig(s));
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         mark(3);
         for (int s = wave; s < S; s += NW) {
@@ -161,7 +170,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
             if (win >= 0) {
                 const uint8_t *src = slot_sig(win);
                 uint8_t *dst = a.sig + (size_t)own_item[o] * a.sig_pitch;
-                for (int b = lane; b < Z::SIG_BYTES; b += 64) dst[b] = src[b];
+                for (int b = lane; b < Sizes<P>::SIG_BYTES; b += 64) dst[b] = src[b];
                 if (lane == 0) {
                     own_done[o] = 1;
                     // attempts the sequential specification makes for this item
