@@ -36,7 +36,7 @@ PEAK_HBM_GBS = 8000.0
 # WRITE_SIZE passes scaled by a 1 GiB copy's readings (read = 2 x FETCH_SIZE,
 # written = WRITE_SIZE); profiles/sign3_16384_hbm_valu_counters.md.
 TRAFFIC_BYTES = {
-    "dilithium3_sign_16384_msg_loguniform_32B-4KiB": (2 * 1760315 + 1334274) * 1024,
+    "dilithium3_sign_16384_msg_loguniform_32B-4KiB": (2 * 1763618 + 1343747) * 1024,
 }
 
 
