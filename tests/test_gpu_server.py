@@ -12,9 +12,10 @@ pytestmark = pytest.mark.gpu
 KEYS = 8
 
 
-def _server(budget=384 << 20, lanes=1, max_batch=128, window=0):
+def _server(budget=384 << 20, lanes=1, max_batch=128, window=0, long_bytes=4096):
     from cudilithium_amd import api
-    srv = api.Server(budget, lanes=lanes, max_batch=max_batch, max_keys=(KEYS, KEYS, KEYS), batch_window_us=window)
+    srv = api.Server(budget, lanes=lanes, max_batch=max_batch, max_keys=(KEYS, KEYS, KEYS), batch_window_us=window,
+                     long_message_bytes=long_bytes)
     keys = {}
     for m in (2, 3, 5):
         seeds = np.random.default_rng(70 + m).integers(0, 256, (KEYS, 32), dtype=np.uint8)
@@ -142,4 +143,23 @@ def test_server_admission_errors():
     assert batch.reqs[2].verdict == -1
     exp, _ = oracle.sign_batch(2, [b"ok"], keys[2][1][1:2])
     assert np.array_equal(batch.out[3], exp[0])
+    srv.close()
+
+
+@pytest.mark.parametrize("long_bytes", [512, 0])
+def test_server_message_classes(long_bytes):
+    """Messages above long_message_bytes go to lanes of their own (512: most
+    of this mix), or every message shares one lane per group (0); results
+    match the oracle either way."""
+    from cudilithium_amd import api
+    srv, keys = _server(budget=256 << 20, lanes=1, max_batch=64, long_bytes=long_bytes)
+    n = 160
+    ops, modes, kidx, msgs, seeds, sigs, _ = _mix(n, keys, 11 + long_bytes, max_mlen=3000)
+    exp = _expected(ops, modes, kidx, msgs, seeds, sigs, keys)
+    batch = api.RequestBatch(ops, modes, kidx, msgs, sigs, seeds)
+    srv.submit(batch.reqs)
+    srv.drain()
+    _check(batch, ops, exp)
+    st = srv.stats()
+    assert st.completed == n and st.rejected == 0
     srv.close()
