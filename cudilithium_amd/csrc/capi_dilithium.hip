@@ -115,6 +115,10 @@ static int sign_dropin(int mode, int engine, uint8_t *sig, size_t sig_pitch, siz
     if (batch > 65536) return DIL_ERANGE;  // (item << 16 | nonce) entries, api.cu:224
     if (sign_pitch < dil_sign_pool_row_bytes(mode, mlen)) return DIL_ENOMEM;
     if (!d_temp_pool || temp_pitch < dil_sign_temp_row_bytes(mode)) return DIL_ENOMEM;
+    // the rows' int32 arrays are read and written 16 bytes per lane (hipMallocPitch pitches qualify)
+    if (sign_pitch % 16 || temp_pitch % 16 || (reinterpret_cast<uintptr_t>(d_sign_pool) & 15) ||
+        (reinterpret_cast<uintptr_t>(d_temp_pool) & 15))
+        return DIL_EINVAL;
     if (!lut->sign_lut || lut->sign_lut_len < batch || !lut->h_exec_lut || !lut->d_exec_lut || !lut->h_done_lut ||
         !lut->d_done_lut || !lut->h_copy_lut || !lut->d_copy_lut)
         return DIL_EINVAL;

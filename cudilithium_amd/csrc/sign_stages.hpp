@@ -137,20 +137,16 @@ __device__ __forceinline__ void stage_expand_a(const SignArgs &a, size_t i, int 
     int32_t *dst = reinterpret_cast<int32_t *>(a.rows + i * a.row_pitch + a.off_ints) + (k * P::L + l) * N;
     // Four consecutive coefficients per 16-byte store: lanes write different
     // rows, so each store instruction touches one cache line per lane, and a
-    // store per coefficient made the kernel wait on its writes.  (A row pitch
-    // that is not a multiple of 16 falls back to 4-byte stores.)
-    if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
-        int4 *dst4 = reinterpret_cast<int4 *>(dst);
-        int32_t b0 = 0, b1 = 0, b2 = 0;  // the last three accepted coefficients
-        uniform_poly(rho, (uint32_t)((k << 8) | l), [&](int ctr, int32_t v) {
-            if ((ctr & 3) == 3) dst4[ctr >> 2] = make_int4(b0, b1, b2, v);
-            b0 = b1;
-            b1 = b2;
-            b2 = v;
-        });
-    } else {
-        uniform_poly(rho, (uint32_t)((k << 8) | l), [&](int ctr, int32_t v) { dst[ctr] = v; });
-    }
+    // store per coefficient made the kernel wait on its writes.  (Row pitches
+    // and the int32 area offset are multiples of 16; the entry points check.)
+    int4 *dst4 = reinterpret_cast<int4 *>(dst);
+    int32_t b0 = 0, b1 = 0, b2 = 0;  // the last three accepted coefficients
+    uniform_poly(rho, (uint32_t)((k << 8) | l), [&](int ctr, int32_t v) {
+        if ((ctr & 3) == 3) dst4[ctr >> 2] = make_int4(b0, b1, b2, v);
+        b0 = b1;
+        b1 = b2;
+        b2 = v;
+    });
 }
 
 // poly p of the item: s1[0..L), s2[0..K), t0[0..K) -> NTT domain (layout D)

@@ -35,3 +35,23 @@ def test_sizes_match_reference_params():
     assert [L.dil_secret_key_bytes(m) for m in (2, 3, 5)] == [2528, 4000, 4864]
     assert [L.dil_signature_bytes(m) for m in (2, 3, 5)] == [2420, 3293, 4595]
     assert L.dil_signature_bytes(4) == 0
+
+
+def test_dropin_sign_rejects_pools_not_16_byte_aligned():
+    """The sign pools' int32 arrays are accessed 16 bytes per lane, so the
+    drop-in rejects pitches (or pool pointers) that are not multiples of 16
+    before any device work -- hipMallocPitch pitches always qualify."""
+    from cudilithium_amd import api
+    L = api.L()
+    mode, batch, mlen = 3, 4, 32
+    row = L.dil_sign_pool_row_bytes(mode, mlen)
+    temp = L.dil_sign_temp_row_bytes(mode)
+    elems = (api.SignLutElement * batch)()
+    fake = ctypes.c_void_p(0x100000)  # never dereferenced: the call returns before device work
+    lut = api.TaskLutC(elems, batch, fake, fake, fake, fake, fake, fake)
+    host = (ctypes.c_uint8 * 64)()
+    siglen = ctypes.c_size_t()
+    for sign_pitch, temp_pitch in ((row + 4, (temp + 15) // 16 * 16), ((row + 15) // 16 * 16, temp + 8)):
+        rc = L.dil_crypto_sign_signature(mode, host, 3293, ctypes.byref(siglen), host, mlen, mlen, host,
+                                         fake, sign_pitch, fake, temp_pitch, ctypes.byref(lut), 2048, batch, None)
+        assert rc == -1, rc  # DIL_EINVAL
