@@ -12,7 +12,7 @@
 //            group's pipeline (persistent sign with resident keys / resident
 //            verify / keygen) on the lane's stream, copy the outputs back in
 //            one transfer and record an event
-//   harvest  wait for the event (blocking in the driver), scatter the
+//   harvest  wait for the event (the thread sleeps until it signals), scatter the
 //            outputs into the requests (signature bytes, verdict, pk || sk),
 //            stamp t_done and publish done = 1
 // Lanes never wait for each other's host work.
@@ -330,9 +330,9 @@ void dil_server::lane_loop(Lane &ln) {
         int e = enqueue(ln);
         stage_ns[ln.op] += mono_ns() - t0;
         ++nb_op[ln.op];
-        // the completion event is a blocking-sync event: the thread sleeps in
-        // the driver instead of polling the runtime (15 polling threads
-        // contend with each other's launches)
+        // the completion event is a blocking-sync event: the thread sleeps
+        // until the event signals instead of polling the runtime (15 polling
+        // threads contend with each other's launches)
         if (!e) e = (int)hipEventSynchronize(ln.ev);
         finish(ln, e);
         lk.lock();
