@@ -28,13 +28,21 @@ HIPFLAGS = [
 ]
 
 
+# Per-file scheduler choice, measured on 16,384-item batches: the iterative
+# ILP scheduler gives the Dilithium5 signing loop +2.6 % signatures/s, while
+# for Dilithium3 (-2 %) and Dilithium2 (-1 %) the default scheduler is faster.
+FILE_FLAGS = {
+    "sign_persistent_5.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+}
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *HIPFLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *HIPFLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     errs = "\n".join(l for l in (r.stderr or "").splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
