@@ -37,6 +37,9 @@ PEAK_HBM_GBS = 8000.0
 # written = WRITE_SIZE); profiles/sign3_16384_hbm_valu_counters.md.
 TRAFFIC_BYTES = {
     "dilithium3_sign_16384_msg_loguniform_32B-4KiB": (2 * 1763618 + 1343747) * 1024,
+    # vf_prep + vf_main + vf_final per call
+    "dilithium5_verify_65536_msg_1KiB_10pct_bitflips": (2 * (142288 + 246351 + 46122) + (77760 + 65462 + 256)) * 1024,
+    "ntt_pipeline_65536x256": (2 * 65576 + 65536) * 1024,
 }
 
 
@@ -245,7 +248,9 @@ def bench_verify(args, mode: int = 5):
                        "engine": "graph", "rejected": int((ret != 0).sum())},
             "eager_value": n * args.steps / eager_wall,
             "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
-                         "frac": achieved / PEAK_INT32_TOPS, "traffic": None, "kernel": "verify pipeline (3 launches)"},
+                         "frac": achieved / PEAK_INT32_TOPS,
+                         "traffic": TRAFFIC_BYTES.get(f"dilithium{mode}_verify_{n}_msg_1KiB_10pct_bitflips"),
+                         "kernel": "verify pipeline (3 launches)"},
             "cpu_baseline": None}
 
 
@@ -275,7 +280,7 @@ def bench_ntt(args):
             "config": {"workload": f"ntt_pipeline_{n}x256", "batch": n},
             "roofline": {"bound": "valu", "achieved": ops / (ms * 1e-3) / 1e12, "peak": PEAK_INT32_TOPS,
                          "unit": "TOP/s", "frac": ops / (ms * 1e-3) / 1e12 / PEAK_INT32_TOPS,
-                         "traffic": None, "hbm_gbs": n * 3072 / (ms * 1e-3) / 1e9},
+                         "traffic": TRAFFIC_BYTES.get(f"ntt_pipeline_{n}x256"), "hbm_gbs": n * 3072 / (ms * 1e-3) / 1e9},
             "cpu_baseline": None}
 
 
