@@ -3,7 +3,10 @@
 // extensions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cudilithium_amd.h"
@@ -29,9 +32,136 @@ auto with_params(int mode, F &&f) {
     }
 }
 
+// Pitched row copy done by the GPU, G bytes per access; one side may be
+// pinned host memory reached through its device mapping.
+template <int G>
+__global__ void __launch_bounds__(256) rows_copy_kernel(uint8_t *dst, size_t dpitch, const uint8_t *src,
+                                                        size_t spitch, size_t wg, size_t total) {
+    using T = typename std::conditional<G == 16, uint4, typename std::conditional<G == 4, uint32_t, uint8_t>::type>::type;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const size_t r = i / wg, c = (i - r * wg) * G;
+        *reinterpret_cast<T *>(dst + r * dpitch + c) = *reinterpret_cast<const T *>(src + r * spitch + c);
+    }
+}
+
+// Where a host-side pointer of a copy lives: pageable memory (unknown to the
+// runtime), pinned host memory (hipHostMalloc) whose allocation holds
+// [p, p + len) -- *view is then its device address -- or anything else.
+enum class HostMem { pageable, pinned, other };
+
+HostMem host_mem(const void *p, size_t len, const uint8_t **view) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory is reported as an error; it is not one here
+        return HostMem::pageable;
+    }
+    if (a.type == hipMemoryTypeUnregistered) return HostMem::pageable;
+    if (a.type != hipMemoryTypeHost || !a.devicePointer) return HostMem::other;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, a.devicePointer) != hipSuccess) {
+        (void)hipGetLastError();
+        return HostMem::other;
+    }
+    const auto *d = static_cast<const uint8_t *>(a.devicePointer), *b = static_cast<const uint8_t *>(base);
+    if (d < b || (size_t)(d - b) > size || len > size - (size_t)(d - b)) return HostMem::other;
+    *view = d;
+    return HostMem::pinned;
+}
+
+// d/s are device addresses (device memory or the device view of pinned host
+// memory); width <= both pitches.
+int rows_copy(uint8_t *d, size_t dpitch, const uint8_t *s, size_t spitch, size_t width, size_t height,
+              hipStream_t st) {
+    const uintptr_t all = reinterpret_cast<uintptr_t>(s) | reinterpret_cast<uintptr_t>(d) | spitch | dpitch | width;
+    const int g = (all & 15) == 0 ? 16 : (all & 3) == 0 ? 4 : 1;
+    const size_t wg = width / g, total = wg * height;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 2048);
+    if (g == 16)
+        rows_copy_kernel<16><<<blocks, 256, 0, st>>>(d, dpitch, s, spitch, wg, total);
+    else if (g == 4)
+        rows_copy_kernel<4><<<blocks, 256, 0, st>>>(d, dpitch, s, spitch, wg, total);
+    else
+        rows_copy_kernel<1><<<blocks, 256, 0, st>>>(d, dpitch, s, spitch, wg, total);
+    return (int)hipGetLastError();
+}
+
+// Pinned staging for pageable host rows, which the GPU cannot reach: the rows
+// are moved in chunks by a CPU memcpy plus the row-copy kernel. One buffer per
+// process (fine-grained, so the GPU never caches a stale chunk), allocated on
+// first use and held until the process ends; one copy uses it at a time.
+constexpr size_t STAGE_BYTES = 32ull << 20;
+std::mutex g_stage_mu;
+uint8_t *g_stage = nullptr, *g_stage_dev = nullptr;
+
+int stage_ready() {
+    if (g_stage) return 0;
+    void *p = nullptr, *d = nullptr;
+    int e = (int)hipHostMalloc(&p, STAGE_BYTES, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e) return e;
+    if ((e = (int)hipHostGetDevicePointer(&d, p, 0))) {
+        (void)hipHostFree(p);
+        return e;
+    }
+    g_stage = static_cast<uint8_t *>(p);
+    g_stage_dev = static_cast<uint8_t *>(d);
+    return 0;
+}
+
+int staged_copy(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height, bool h2d,
+                hipStream_t st) {
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    int e = stage_ready();
+    if (e) return e;
+    const size_t rows_per = STAGE_BYTES / width;
+    for (size_t r0 = 0; r0 < height; r0 += rows_per) {
+        const size_t nr = std::min(rows_per, height - r0);
+        if (h2d) {
+            const uint8_t *hs = static_cast<const uint8_t *>(src) + r0 * spitch;
+            if (spitch == width)
+                std::memcpy(g_stage, hs, nr * width);
+            else
+                for (size_t i = 0; i < nr; ++i) std::memcpy(g_stage + i * width, hs + i * spitch, width);
+            if ((e = rows_copy(static_cast<uint8_t *>(dst) + r0 * dpitch, dpitch, g_stage_dev, width, width, nr, st)))
+                return e;
+            if ((e = (int)hipStreamSynchronize(st))) return e;  // the next chunk reuses the buffer
+        } else {
+            if ((e = rows_copy(g_stage_dev, width, static_cast<const uint8_t *>(src) + r0 * spitch, spitch, width, nr,
+                               st)))
+                return e;
+            if ((e = (int)hipStreamSynchronize(st))) return e;
+            uint8_t *hd = static_cast<uint8_t *>(dst) + r0 * dpitch;
+            if (dpitch == width)
+                std::memcpy(hd, g_stage, nr * width);
+            else
+                for (size_t i = 0; i < nr; ++i) std::memcpy(hd + i * dpitch, g_stage + i * width, width);
+        }
+    }
+    return 0;
+}
+
+// Host <-> pitched device rows. hipMemcpy2DAsync moves a host-side 2D copy one
+// row at a time (about 2 us per row: ~100 ms for 16,384 Dil3 signing rows from
+// pinned memory, ~400 ms from pageable), so the GPU copies the rows itself:
+// directly from / to pinned host memory (asynchronous, like the runtime copy),
+// through the staging buffer for pageable memory (returns once copied, as the
+// runtime's pageable copies do).
 int copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height, hipMemcpyKind k,
            hipStream_t st) {
     if (width == 0 || height == 0) return 0;
+    if ((k == hipMemcpyHostToDevice || k == hipMemcpyDeviceToHost) && width <= spitch && width <= dpitch) {
+        const bool h2d = k == hipMemcpyHostToDevice;
+        const size_t hpitch = h2d ? spitch : dpitch;
+        const uint8_t *hv = nullptr;
+        const HostMem hm = host_mem(h2d ? src : dst, (height - 1) * hpitch + width, &hv);
+        if (hm == HostMem::pinned) {
+            if (h2d) return rows_copy(static_cast<uint8_t *>(dst), dpitch, hv, spitch, width, height, st);
+            return rows_copy(const_cast<uint8_t *>(hv), dpitch, static_cast<const uint8_t *>(src), spitch, width,
+                             height, st);
+        }
+        if (hm == HostMem::pageable && width <= STAGE_BYTES)
+            return staged_copy(dst, dpitch, src, spitch, width, height, h2d, st);
+    }
     return (int)hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, k, st);
 }
 
