@@ -165,6 +165,13 @@ int copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t widt
     return (int)hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, k, st);
 }
 
+// Kind of a copy between the library's device rows and a caller pointer that
+// may be host or device memory (the key store's load and store).
+hipMemcpyKind caller_kind(const void *caller, size_t len, hipMemcpyKind host_kind) {
+    const uint8_t *v = nullptr;
+    return host_mem(caller, len, &v) == HostMem::other ? hipMemcpyDefault : host_kind;
+}
+
 }  // namespace
 
 static_assert(sizeof(dil_sign_lut_element) == sizeof(SignLutEntry), "sign_lut_element layout");
@@ -815,8 +822,10 @@ int dil_keystore_load(dil_keystore *k, size_t first, const uint8_t *pk, size_t p
         using KY = KeyLayout<P>;
         uint8_t *row = k->rows + first * k->pitch;
         int e;
-        if ((e = copy2d(row + KY::PK, k->pitch, pk, pk_pitch, Z::PK_BYTES, n, hipMemcpyDefault, st))) return e;
-        if (sk && (e = copy2d(row + KY::SK, k->pitch, sk, sk_pitch, Z::SK_BYTES, n, hipMemcpyDefault, st)))
+        if ((e = copy2d(row + KY::PK, k->pitch, pk, pk_pitch, Z::PK_BYTES, n,
+                          caller_kind(pk, (n - 1) * pk_pitch + Z::PK_BYTES, hipMemcpyHostToDevice), st))) return e;
+        if (sk && (e = copy2d(row + KY::SK, k->pitch, sk, sk_pitch, Z::SK_BYTES, n,
+                                caller_kind(sk, (n - 1) * sk_pitch + Z::SK_BYTES, hipMemcpyHostToDevice), st)))
             return e;
         return launch_key_expand(k->mode, row, k->pitch, n, sk ? 1 : 0, st);
     });
@@ -854,8 +863,10 @@ int dil_keystore_export(const dil_keystore *k, size_t first, size_t n, uint8_t *
         using KY = KeyLayout<P>;
         const uint8_t *row = k->rows + first * k->pitch;
         int e;
-        if (pk && (e = copy2d(pk, pk_pitch, row + KY::PK, k->pitch, Z::PK_BYTES, n, hipMemcpyDefault, st))) return e;
-        if (sk && (e = copy2d(sk, sk_pitch, row + KY::SK, k->pitch, Z::SK_BYTES, n, hipMemcpyDefault, st))) return e;
+        if (pk && (e = copy2d(pk, pk_pitch, row + KY::PK, k->pitch, Z::PK_BYTES, n,
+                                caller_kind(pk, (n - 1) * pk_pitch + Z::PK_BYTES, hipMemcpyDeviceToHost), st))) return e;
+        if (sk && (e = copy2d(sk, sk_pitch, row + KY::SK, k->pitch, Z::SK_BYTES, n,
+                                caller_kind(sk, (n - 1) * sk_pitch + Z::SK_BYTES, hipMemcpyDeviceToHost), st))) return e;
         return 0;
     });
 }
