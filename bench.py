@@ -9,10 +9,10 @@ Printed on one JSON line: throughput (signatures/s, the persistent engine
 replayed as a HIP graph), the roofline of the dominant kernel (the persistent
 rejection loop, integer-VALU bound), the oracle's CPU rate on a bounded
 sample, the per-stage (reference architecture) GPU form on the same inputs,
-and a serving summary (resident keys, continuous batching: p50 / p99 latency
-at two offered loads).
+a serving summary (resident keys, continuous batching: p50 / p99 latency
+at two offered loads) and the drop-in call's rate with host buffers.
 
-    python bench.py [--steps K] [--warmup W] [--workload sign3|sign2|sign5|verify5|ntt|copy|serve]
+    python bench.py [--steps K] [--warmup W] [--workload sign3|sign2|sign5|verify5|ntt|copy|serve|dropin]
 """
 from __future__ import annotations
 
@@ -171,6 +171,60 @@ def bench_sign(args, mode: int):
         "cpu_baseline": cpu, "perstage_gpu_baseline": perstage, "eager_persistent_value": eager_value,
         "speedup_vs_perstage": (value / perstage["value"]) if perstage else None,
     }
+
+
+def bench_dropin(args, mode: int = 3):
+    """The reference's call surface with host buffers (api.cu:99-447,
+    crypto_sign_signature): sk, messages and signatures cross PCIe inside
+    every call, so this is the PCIe-inclusive rate, never the headline."""
+    from cudilithium_amd import api, workloads
+    from cudilithium_amd.device import HostBuffer
+
+    n, mlen = args.batch or 16384, 64
+    seeds, _ = workloads.config3(n) if mode == 3 else (
+        workloads.xoshiro_bytes(workloads.CONFIG_SEED[3] + mode, n * 32).reshape(n, 32), None)
+    pkb, skb, sigb = api.sizes(mode)
+    _, sk = api.keypair_device(mode, seeds=seeds)
+    sk = np.ascontiguousarray(sk)
+    m = workloads.xoshiro_bytes(workloads.CONFIG_SEED[3] + 77, n * mlen).reshape(n, mlen)
+    pool = api.reference_pool(mode, api.POOL_SIGN, n, mlen)
+    hsk, hm, hsig = HostBuffer((n, skb), np.uint8), HostBuffer((n, mlen), np.uint8), HostBuffer((n, sigb), np.uint8)
+    hsk.array[...] = sk
+    hm.array[...] = m
+    out, sigs = {}, {}
+    reps = max(1, min(args.steps, 5))
+    for host in ("pinned", "pageable"):
+        for thr in (n, 2048):
+            if host == "pageable" and thr != n:
+                continue
+            temp = api.reference_pool(mode, api.POOL_SIGN_TEMP, thr)
+            lut = api.TaskLut(n, thr)
+            s_, m_, sig = (hsk.array, hm.array, hsig.array) if host == "pinned" else (sk, m, np.zeros((n, sigb), np.uint8))
+
+            def call():
+                api.crypto_sign_signature(mode, sig, sigb, m_, mlen, mlen, s_, pool, pool.pitch, temp, temp.pitch,
+                                          lut, thr, n)
+                api.stream_sync(None)
+            call()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                call()
+            t = (time.perf_counter() - t0) / reps
+            key = f"{host}_exec_threshold_{thr}"
+            out[key] = {"value": n / t, "ms_per_call": t * 1e3}
+            sigs[key] = sig.copy()
+            log(f"[bench] drop-in {key}: {t * 1e3:.2f} ms per call")
+    import oracle
+    idx = np.arange(0, n, max(1, n // min(n, args.check)))[:args.check]
+    exp, _ = oracle.sign_batch(mode, [bytes(m[i]) for i in idx], sk[idx])
+    first = next(iter(sigs.values()))
+    parity = all(np.array_equal(s, first) for s in sigs.values()) and bool(np.array_equal(first[idx], exp))
+    log(f"[bench] drop-in parity (all paths equal, {len(idx)} items vs oracle): {parity}")
+    best = max(out.values(), key=lambda r: r["value"])
+    return {"metric": METRIC, "value": best["value"], "unit": "signs/s", "ms_per_step": best["ms_per_call"],
+            "dtype": "int32", "parity": "bit-exact" if parity else "MISMATCH",
+            "config": {"workload": f"dilithium{mode}_dropin_sign_{n}_msg_64B_host_buffers", "batch": n,
+                       "mode": mode, "calls": out}}
 
 
 def cpu_baseline_sign(mode, msgs, sk, seconds):
@@ -466,6 +520,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-perstage", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="sign3: skip the serving summary")
+    ap.add_argument("--no-dropin", action="store_true", help="sign3: skip the host-buffer drop-in summary")
     ap.add_argument("--requests", type=int, default=40000, help="serve: requests per run")
     ap.add_argument("--budget-gib", type=float, default=4.0, help="serve: HBM budget of the in-flight batches")
     ap.add_argument("--lanes", type=int, default=1, help="serve: batches in flight per (op, mode) group")
@@ -500,6 +555,15 @@ def main():
                                                            "p99_us_long", "long_fraction")} for r in srv["loads"]]}
             if srv["parity"] != "bit-exact":
                 res["parity"] = "MISMATCH"
+        if w == "sign3" and not args.no_dropin:
+            # the reference's host-buffer call surface on the same keys (PCIe-inclusive)
+            dr = bench_dropin(args, 3)
+            res["dropin_host_buffers"] = {"workload": dr["config"]["workload"], "parity": dr["parity"],
+                                          "unit": "signs/s", "calls": dr["config"]["calls"]}
+            if dr["parity"] != "bit-exact":
+                res["parity"] = "MISMATCH"
+    elif w.startswith("dropin"):
+        res = bench_dropin(args, int(w[6:] or 3))
     elif w.startswith("verify"):
         res = bench_verify(args, int(w[6:] or 5))
     elif w == "ntt":
