@@ -289,12 +289,24 @@ static int sign_dropin(int mode, int engine, uint8_t *sig, size_t sig_pitch, siz
                               lut->h_done_lut, lut->d_done_lut, reinterpret_cast<uint64_t *>(lut->h_copy_lut),
                               reinterpret_cast<uint64_t *>(lut->d_copy_lut), st);
         } else {
-            // the exec_threshold temp rows are the attempt slots; the first
-            // word of d_copy_lut is the work-queue head
+            // Attempt slots. Up to exec_threshold items: the temp rows. Larger
+            // batches: slot s keeps its state in sign-pool row s (its slot area,
+            // otherwise unused here) and its candidate signature in the temp pool
+            // read as one array of SIG_STRIDE entries, so the same pools give up
+            // to `batch` slots (a card-filling batch at the reference's default
+            // exec_threshold of 2048). Results do not depend on the slot count.
+            // The first word of d_copy_lut is the work-queue head.
             if ((e = launch_sign_fixed(mode, a, st))) return e;
+            constexpr size_t SIG_STRIDE = align256(Z::SIG_BYTES);
+            SlotRef slots = temp_slots;
+            size_t nslots = exec_threshold;
+            if (batch > exec_threshold) {
+                nslots = std::min(batch, exec_threshold * temp_pitch / SIG_STRIDE);
+                slots = SlotRef{d_sign_pool + SLy::SLOT, sign_pitch, d_temp_pool, SIG_STRIDE};
+            }
             uint32_t *ctl = reinterpret_cast<uint32_t *>(lut->d_copy_lut);
-            e = launch_sign_persistent(mode, a, temp_slots, exec_threshold, default_slots_per_block(exec_threshold),
-                                       0, 0, 0, ctl, nullptr, st);
+            e = launch_sign_persistent(mode, a, slots, nslots, default_slots_per_block(nslots), 0, 0, 0, ctl, nullptr,
+                                       st);
             if (e == -3) return DIL_ERANGE;
             if (e) return e;
             // like the reference's call, return once the rejection loop is done
