@@ -184,8 +184,8 @@ def bench_dropin(args, mode: int = 3):
     seeds, _ = workloads.config3(n) if mode == 3 else (
         workloads.xoshiro_bytes(workloads.CONFIG_SEED[3] + mode, n * 32).reshape(n, 32), None)
     pkb, skb, sigb = api.sizes(mode)
-    _, sk = api.keypair_device(mode, seeds=seeds)
-    sk = np.ascontiguousarray(sk)
+    pk, sk = api.keypair_device(mode, seeds=seeds)
+    pk, sk = np.ascontiguousarray(pk), np.ascontiguousarray(sk)
     m = workloads.xoshiro_bytes(workloads.CONFIG_SEED[3] + 77, n * mlen).reshape(n, mlen)
     pool = api.reference_pool(mode, api.POOL_SIGN, n, mlen)
     hsk, hm, hsig = HostBuffer((n, skb), np.uint8), HostBuffer((n, mlen), np.uint8), HostBuffer((n, sigb), np.uint8)
@@ -219,8 +219,32 @@ def bench_dropin(args, mode: int = 3):
     exp, _ = oracle.sign_batch(mode, [bytes(m[i]) for i in idx], sk[idx])
     first = next(iter(sigs.values()))
     parity = all(np.array_equal(s, first) for s in sigs.values()) and bool(np.array_equal(first[idx], exp))
-    log(f"[bench] drop-in parity (all paths equal, {len(idx)} items vs oracle): {parity}")
-    best = max(out.values(), key=lambda r: r["value"])
+    # crypto_sign_verify (api.cu:455-515) on the signatures, one flipped bit in every 10th
+    bad = first.copy()
+    bad[::10, 7] ^= 1
+    want = np.zeros(n, np.int32)
+    want[::10] = -1
+    vpool = api.reference_pool(mode, api.POOL_VERIFY, n, mlen)
+    hpk, hbad, hret = HostBuffer((n, pkb), np.uint8), HostBuffer((n, sigb), np.uint8), HostBuffer((n,), np.int32)
+    hpk.array[...] = pk
+    hbad.array[...] = bad
+    for host in ("pinned", "pageable"):
+        p_, s_, m_, ret = (hpk.array, hbad.array, hm.array, hret.array) if host == "pinned" else (
+            pk, bad, m, np.zeros(n, np.int32))
+
+        def vcall():
+            api.crypto_sign_verify(mode, ret, s_, sigb, sigb, m_, mlen, mlen, p_, vpool, vpool.pitch, n)
+            api.stream_sync(None)
+        vcall()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            vcall()
+        t = (time.perf_counter() - t0) / reps
+        out[f"verify_{host}"] = {"value": n / t, "unit": "verifies/s", "ms_per_call": t * 1e3}
+        parity = parity and bool(np.array_equal(ret, want))
+        log(f"[bench] drop-in verify {host}: {t * 1e3:.2f} ms per call")
+    log(f"[bench] drop-in parity (sign paths equal, {len(idx)} items vs oracle; verdicts): {parity}")
+    best = max((r for k, r in out.items() if not k.startswith("verify")), key=lambda r: r["value"])
     return {"metric": METRIC, "value": best["value"], "unit": "signs/s", "ms_per_step": best["ms_per_call"],
             "dtype": "int32", "parity": "bit-exact" if parity else "MISMATCH",
             "config": {"workload": f"dilithium{mode}_dropin_sign_{n}_msg_64B_host_buffers", "batch": n,
