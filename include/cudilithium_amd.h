@@ -96,6 +96,12 @@ size_t dil_sign_temp_row_bytes(int mode);
 size_t dil_verify_pool_row_bytes(int mode, size_t mlen);
 size_t dil_reference_pool_row_bytes(int mode, int which, size_t mlen);
 
+/* Host buffers of the three drop-in calls may be pinned (hipHostMalloc) or
+ * pageable.  Pinned rows are copied by a kernel on `stream` through their
+ * device mapping, asynchronously; pageable rows go through a 32 MiB pinned
+ * staging buffer that the library allocates on first use and keeps until the
+ * process ends, and those copies complete before the call returns. */
+
 /* crypto_sign_keypair (api.cuh:27-29, api.cu:26-61): batch keys whose seeds
  * are SHAKE128(le64(rand_index + 2i + 1))[0:32]; pk/sk are host buffers
  * (packed, stride CRYPTO_PUBLICKEYBYTES / CRYPTO_SECRETKEYBYTES), copied back
@@ -109,8 +115,10 @@ int dil_crypto_sign_keypair(int mode, uint8_t *pk, uint8_t *sk, uint8_t *d_keypa
  * host secret keys; signatures to host `sig` with stride sig_pitch;
  * *siglen = CRYPTO_BYTES.  Per item the result is the attempt with the first
  * successful nonce, as in the reference.  This entry point runs the
- * persistent work-queue engine; exec_threshold is the number of attempt
- * slots (temp pool rows).  batch_size <= 65536. */
+ * persistent work-queue engine.  Attempt slots: the exec_threshold temp
+ * rows when batch_size <= exec_threshold; otherwise up to batch_size slots,
+ * each with its state in a sign-pool row and its candidate signature in the
+ * temp pool read as one array.  batch_size <= 65536. */
 int dil_crypto_sign_signature(int mode, uint8_t *sig, size_t sig_pitch, size_t *siglen,
                               const uint8_t *m, size_t m_pitch, size_t mlen, const uint8_t *sk,
                               uint8_t *d_sign_mem_pool, size_t d_sign_mem_pool_pitch,
