@@ -87,10 +87,12 @@ int rows_copy(uint8_t *d, size_t dpitch, const uint8_t *s, size_t spitch, size_t
 }
 
 // Pinned staging for pageable host rows, which the GPU cannot reach: the rows
-// are moved in chunks by a CPU memcpy plus the row-copy kernel. One buffer per
-// process (fine-grained, so the GPU never caches a stale chunk), allocated on
-// first use and held until the process ends; one copy uses it at a time.
-constexpr size_t STAGE_BYTES = 32ull << 20;
+// are moved in chunks by a CPU memcpy plus the row-copy kernel, alternating
+// between the buffer's two halves so that the CPU fills (or drains) one while
+// the GPU drains (or fills) the other. One buffer per process (fine-grained,
+// so the GPU never caches a stale chunk), allocated on first use and held
+// until the process ends; one copy uses it at a time.
+constexpr size_t STAGE_BYTES = 32ull << 20, STAGE_HALF = STAGE_BYTES / 2;
 std::mutex g_stage_mu;
 uint8_t *g_stage = nullptr, *g_stage_dev = nullptr;
 
@@ -108,34 +110,46 @@ int stage_ready() {
     return 0;
 }
 
+// nr rows of width bytes from `from` (pitch fp) to `to` (pitch tp)
+void pack_rows(uint8_t *to, size_t tp, const uint8_t *from, size_t fp, size_t width, size_t nr) {
+    if (tp == width && fp == width)
+        std::memcpy(to, from, nr * width);
+    else
+        for (size_t i = 0; i < nr; ++i) std::memcpy(to + i * tp, from + i * fp, width);
+}
+
 int staged_copy(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height, bool h2d,
                 hipStream_t st) {
     std::lock_guard<std::mutex> lock(g_stage_mu);
     int e = stage_ready();
     if (e) return e;
-    const size_t rows_per = STAGE_BYTES / width;
-    for (size_t r0 = 0; r0 < height; r0 += rows_per) {
-        const size_t nr = std::min(rows_per, height - r0);
-        if (h2d) {
-            const uint8_t *hs = static_cast<const uint8_t *>(src) + r0 * spitch;
-            if (spitch == width)
-                std::memcpy(g_stage, hs, nr * width);
-            else
-                for (size_t i = 0; i < nr; ++i) std::memcpy(g_stage + i * width, hs + i * spitch, width);
-            if ((e = rows_copy(static_cast<uint8_t *>(dst) + r0 * dpitch, dpitch, g_stage_dev, width, width, nr, st)))
+    const size_t rows_per = STAGE_HALF / width, nchunks = (height + rows_per - 1) / rows_per;
+    auto rows_of = [&](size_t k) { return std::min(rows_per, height - k * rows_per); };
+    auto host_half = [&](size_t k) { return g_stage + (k & 1) * STAGE_HALF; };
+    auto dev_half = [&](size_t k) { return g_stage_dev + (k & 1) * STAGE_HALF; };
+    if (h2d) {
+        const auto *hs = static_cast<const uint8_t *>(src);
+        auto *dd = static_cast<uint8_t *>(dst);
+        for (size_t k = 0; k < nchunks; ++k) {
+            // half k & 1 last held chunk k - 2, whose copy the previous
+            // iteration's synchronisation has seen finish
+            pack_rows(host_half(k), width, hs + k * rows_per * spitch, spitch, width, rows_of(k));
+            if (k && (e = (int)hipStreamSynchronize(st))) return e;
+            if ((e = rows_copy(dd + k * rows_per * dpitch, dpitch, dev_half(k), width, width, rows_of(k), st)))
                 return e;
-            if ((e = (int)hipStreamSynchronize(st))) return e;  // the next chunk reuses the buffer
-        } else {
-            if ((e = rows_copy(g_stage_dev, width, static_cast<const uint8_t *>(src) + r0 * spitch, spitch, width, nr,
-                               st)))
-                return e;
-            if ((e = (int)hipStreamSynchronize(st))) return e;
-            uint8_t *hd = static_cast<uint8_t *>(dst) + r0 * dpitch;
-            if (dpitch == width)
-                std::memcpy(hd, g_stage, nr * width);
-            else
-                for (size_t i = 0; i < nr; ++i) std::memcpy(hd + i * dpitch, g_stage + i * width, width);
         }
+        return (int)hipStreamSynchronize(st);  // the next user may overwrite the buffer
+    }
+    const auto *ds = static_cast<const uint8_t *>(src);
+    auto *hd = static_cast<uint8_t *>(dst);
+    for (size_t k = 0; k <= nchunks; ++k) {
+        // chunk k - 1 is complete once the stream has drained; then the GPU
+        // fills the other half with chunk k while the CPU unpacks chunk k - 1
+        if (k && (e = (int)hipStreamSynchronize(st))) return e;
+        if (k < nchunks &&
+            (e = rows_copy(dev_half(k), width, ds + k * rows_per * spitch, spitch, width, rows_of(k), st)))
+            return e;
+        if (k) pack_rows(hd + (k - 1) * rows_per * dpitch, dpitch, host_half(k - 1), width, width, rows_of(k - 1));
     }
     return 0;
 }
@@ -159,7 +173,7 @@ int copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t widt
             return rows_copy(const_cast<uint8_t *>(hv), dpitch, static_cast<const uint8_t *>(src), spitch, width,
                              height, st);
         }
-        if (hm == HostMem::pageable && width <= STAGE_BYTES)
+        if (hm == HostMem::pageable && width <= STAGE_HALF)
             return staged_copy(dst, dpitch, src, spitch, width, height, h2d, st);
     }
     return (int)hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, k, st);

@@ -5,7 +5,8 @@ its pinned staging buffer; the C++ drop-in test pins the pageable path to the
 reference's KAT hashes. Every copy of api.cu:55-58, 117-126, 443-446 and
 504-513 is covered, with 16-, 4- and 1-byte access widths, an odd host pitch,
 a pointer that starts inside its allocation, a pinned buffer rewritten by the
-CPU between calls, and pageable copies larger than the 32 MiB staging buffer."""
+CPU between calls, and pageable copies over several 16 MiB staging chunks
+(the staging buffer's two halves in turn)."""
 import numpy as np
 import pytest
 
@@ -84,7 +85,7 @@ def test_dropin_pinned_host_buffers_match_pageable(mode):
 
 
 def test_dropin_pageable_copies_span_staging_chunks():
-    mode, n, mlen, thr = 3, 8500, 16, 2048  # sk rows 4000 B: 34 MB, two staging chunks
+    mode, n, mlen, thr = 3, 8500, 16, 2048  # sk rows 4000 B: 34 MB, three 16 MiB chunks
     pkb, skb, sigb = api.sizes(mode)
     kp = api.reference_pool(mode, api.POOL_KEYPAIR, n)
     pk0, sk0 = np.zeros((n, pkb), np.uint8), np.zeros((n, skb), np.uint8)
