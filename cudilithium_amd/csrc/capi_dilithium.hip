@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -110,12 +111,35 @@ int stage_ready() {
     return 0;
 }
 
-// nr rows of width bytes from `from` (pitch fp) to `to` (pitch tp)
-void pack_rows(uint8_t *to, size_t tp, const uint8_t *from, size_t fp, size_t width, size_t nr) {
+// nr rows of width bytes from `from` (pitch fp) to `to` (pitch tp); chunks of
+// 8 MiB and more are split over PACK_THREADS host threads (one core's memcpy
+// runs well below what the host memory and the GPU copy sustain)
+constexpr unsigned PACK_THREADS = 4;
+
+void pack_rows_serial(uint8_t *to, size_t tp, const uint8_t *from, size_t fp, size_t width, size_t nr) {
     if (tp == width && fp == width)
         std::memcpy(to, from, nr * width);
     else
         for (size_t i = 0; i < nr; ++i) std::memcpy(to + i * tp, from + i * fp, width);
+}
+
+void pack_rows(uint8_t *to, size_t tp, const uint8_t *from, size_t fp, size_t width, size_t nr) {
+    if (nr * width < (8u << 20) || nr < PACK_THREADS) return pack_rows_serial(to, tp, from, fp, width, nr);
+    auto part = [&](unsigned t) {
+        const size_t r0 = nr * t / PACK_THREADS, r1 = nr * (t + 1) / PACK_THREADS;
+        pack_rows_serial(to + r0 * tp, tp, from + r0 * fp, fp, width, r1 - r0);
+    };
+    std::thread th[PACK_THREADS - 1];
+    for (unsigned t = 1; t < PACK_THREADS; ++t) {
+        try {
+            th[t - 1] = std::thread(part, t);
+        } catch (...) {
+            part(t);  // no thread available: copy this part here
+        }
+    }
+    part(0);
+    for (auto &x : th)
+        if (x.joinable()) x.join();
 }
 
 int staged_copy(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height, bool h2d,

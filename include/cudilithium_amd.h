@@ -100,7 +100,8 @@ size_t dil_reference_pool_row_bytes(int mode, int which, size_t mlen);
  * pageable.  Pinned rows are copied by a kernel on `stream` through their
  * device mapping, asynchronously; pageable rows go through a 32 MiB pinned
  * staging buffer that the library allocates on first use and keeps until the
- * process ends, and those copies complete before the call returns. */
+ * process ends (packed by up to 4 host threads), and those copies complete
+ * before the call returns. */
 
 /* crypto_sign_keypair (api.cuh:27-29, api.cu:26-61): batch keys whose seeds
  * are SHAKE128(le64(rand_index + 2i + 1))[0:32]; pk/sk are host buffers
