@@ -20,6 +20,8 @@ u64p, u32p, i32p = ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint
 
 POOL_KEYPAIR, POOL_SIGN, POOL_SIGN_TEMP, POOL_VERIFY = 0, 1, 2, 3
 ENGINE_PERSISTENT, ENGINE_PERSTAGE = 0, 1
+# dil_sign_ctx_status bits
+STATUS_ROUND_CAP, STATUS_NO_SECRET = 1, 2
 
 
 class SignLutElement(ctypes.Structure):  # api.cuh:31-34
@@ -81,6 +83,7 @@ def _register():
     R("dil_keystore_destroy", c_int, [c_void_p])
     R("dil_keystore_capacity", c_size_t, [c_void_p])
     R("dil_keystore_mode", c_int, [c_void_p])
+    R("dil_keystore_has_secret", c_int, [c_void_p, c_size_t])
     R("dil_keystore_load", c_int, [c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t, c_void_p])
     R("dil_keystore_keygen", c_int, [c_void_p, c_size_t, c_void_p, c_size_t, ctypes.c_uint64, c_size_t, c_void_p])
     R("dil_keystore_export", c_int, [c_void_p, c_size_t, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p])
@@ -345,8 +348,11 @@ class SignContext:
         d_idx = DeviceBuffer.from_array(np.ascontiguousarray(key_idx, dtype=np.uint32))
         d_sig = DeviceBuffer(n * self.sig_bytes)
         self.sign_resident_device(store, d_idx, d_sig, dm, engine)
-        if engine == ENGINE_PERSISTENT and self.status() != 0:
+        st = self.status()
+        if st & STATUS_ROUND_CAP:
             raise _lib.DilError("persistent engine stopped at its round limit")
+        if st & STATUS_NO_SECRET:
+            raise _lib.DilError("a key of the batch has no secret key (verify-only)")
         return d_sig.download((n, self.sig_bytes), np.uint8)
 
     def sign_randomized(self, msgs, rhoprime: np.ndarray, sk: np.ndarray | None = None,
@@ -363,8 +369,11 @@ class SignContext:
                                                   d_idx.ptr if d_idx else None, d_sig.ptr, self.sig_bytes,
                                                   d_sk.ptr if d_sk else None, self.sk_bytes, dm.data.ptr, dm.offs.ptr,
                                                   dm.lens.ptr, d_rnd.ptr, 64, n, engine, None), "sign_randomized")
-        if engine == ENGINE_PERSISTENT and self.status() != 0:
+        st = self.status()
+        if st & STATUS_ROUND_CAP:
             raise _lib.DilError("persistent engine stopped at its round limit")
+        if st & STATUS_NO_SECRET:
+            raise _lib.DilError("a key of the batch has no secret key (verify-only)")
         return d_sig.download((n, self.sig_bytes), np.uint8)
 
     def graph(self, d_sig: DeviceBuffer, d_sk: DeviceBuffer, msgs: DeviceMessages) -> "Graph":
@@ -486,13 +495,17 @@ class KeyStore:
                                            None), "keystore_keygen")
         stream_sync(None)
 
-    def export(self, first: int, n: int):
+    def export(self, first: int, n: int, with_sk: bool = True):
         pk = np.zeros((n, self.pk_bytes), np.uint8)
-        sk = np.zeros((n, self.sk_bytes), np.uint8)
-        _lib.check(L().dil_keystore_export(self.handle, first, n, pk.ctypes.data, self.pk_bytes, sk.ctypes.data,
-                                           self.sk_bytes, None), "keystore_export")
+        sk = np.zeros((n, self.sk_bytes), np.uint8) if with_sk else None
+        _lib.check(L().dil_keystore_export(self.handle, first, n, pk.ctypes.data, self.pk_bytes,
+                                           sk.ctypes.data if with_sk else None, self.sk_bytes, None),
+                   "keystore_export")
         stream_sync(None)
         return pk, sk
+
+    def has_secret(self, idx: int) -> bool:
+        return bool(L().dil_keystore_has_secret(self.handle, idx))
 
     def verify(self, key_idx, sig: np.ndarray, msgs) -> np.ndarray:
         n = len(msgs)

@@ -147,6 +147,16 @@ int staged_copy(void *dst, size_t dpitch, const void *src, size_t spitch, size_t
     std::lock_guard<std::mutex> lock(g_stage_mu);
     int e = stage_ready();
     if (e) return e;
+    // on an early return a copy kernel may still be queued on the stream with
+    // a staging half as its source or destination: drain the stream before
+    // the lock is released (its result is the error already being returned)
+    struct Drain {
+        hipStream_t st;
+        bool armed = true;
+        ~Drain() {
+            if (armed) (void)hipStreamSynchronize(st);
+        }
+    } drain{st};
     const size_t rows_per = STAGE_HALF / width, nchunks = (height + rows_per - 1) / rows_per;
     auto rows_of = [&](size_t k) { return std::min(rows_per, height - k * rows_per); };
     auto host_half = [&](size_t k) { return g_stage + (k & 1) * STAGE_HALF; };
@@ -162,6 +172,7 @@ int staged_copy(void *dst, size_t dpitch, const void *src, size_t spitch, size_t
             if ((e = rows_copy(dd + k * rows_per * dpitch, dpitch, dev_half(k), width, width, rows_of(k), st)))
                 return e;
         }
+        drain.armed = false;
         return (int)hipStreamSynchronize(st);  // the next user may overwrite the buffer
     }
     const auto *ds = static_cast<const uint8_t *>(src);
@@ -175,6 +186,7 @@ int staged_copy(void *dst, size_t dpitch, const void *src, size_t spitch, size_t
             return e;
         if (k) pack_rows(hd + (k - 1) * rows_per * dpitch, dpitch, host_half(k - 1), width, width, rows_of(k - 1));
     }
+    drain.armed = false;  // the last iteration synchronised after the last copy
     return 0;
 }
 
@@ -390,6 +402,7 @@ int dil_crypto_sign_verify(int mode, int *ret, const uint8_t *sig, size_t sig_pi
     }
     if (!sig || !pk || !d_pool || (mlen && !m)) return DIL_EINVAL;
     if (pitch < dil_verify_pool_row_bytes(mode, mlen)) return DIL_ENOMEM;
+    if (pitch % 4) return DIL_EINVAL;  // the verdict is an int32 in each row
     return with_params(mode, [&](auto p) -> int {
         using P = decltype(p);
         using VL = VerifyLayout<P>;
@@ -405,7 +418,6 @@ int dil_crypto_sign_verify(int mode, int *ret, const uint8_t *sig, size_t sig_pi
         VerifyArgs a{d_pool + VL::SIG, pitch, d_pool + VL::PK, pitch,
                      MsgRef{d_pool + VL::M, nullptr, pitch, nullptr, (uint32_t)mlen},
                      d_pool, pitch, d_ret, pitch / 4, batch};
-        if (pitch % 4) return DIL_EINVAL;
         if ((e = launch_verify(mode, a, st))) return e;
         // api.cu:513
         return copy2d(ret, sizeof(int), d_ret, pitch, sizeof(int), batch, hipMemcpyDeviceToHost, st);
@@ -505,6 +517,8 @@ struct dil_keystore {
     int mode = 0;
     size_t capacity = 0, pitch = 0;
     uint8_t *rows = nullptr;
+    // host copy of each key's KEY_HAS_SECRET flag, set by load / keygen calls
+    std::vector<uint8_t> has_sk;
 };
 
 // sign n items; with ks, item i uses resident key d_key_idx[i] (d_sk unused)
@@ -553,7 +567,9 @@ static int sign_enqueue(dil_sign_ctx *c, uint8_t *d_sig, size_t sig_pitch, const
         hipEvent_t *e3 = timing && c->timed < c->timing_cap ? &c->ev[3 * c->timed] : nullptr;
         if (e3 && (e = (int)hipEventRecord(e3[0], st))) return e;
         if (engine == DIL_ENGINE_PERSTAGE) {
-            // the per-stage scheduler launches the fixed part itself
+            // the per-stage scheduler launches the fixed part itself; the
+            // status word is cleared here (the persistent launch clears it)
+            if ((e = launch_zero_ctl(c->d_head, st))) return e;
             if (e3 && (e = (int)hipEventRecord(e3[1], st))) return e;
             const SlotRef items{c->rows + SignLayout<P>::SLOT, c->row_pitch, d_sig, sig_pitch};
             e = sign_perstage(c->mode, a, items, slots, c->nslots, c->sign_lut.data(), c->h_exec, c->d_exec,
@@ -569,6 +585,7 @@ static int sign_enqueue(dil_sign_ctx *c, uint8_t *d_sig, size_t sig_pitch, const
             if (e == -3) return DIL_ERANGE;
         }
         if (e) return e;
+        if (ks && (e = launch_sign_key_check(c->mode, a, c->d_head + 1, st))) return e;
         if (e3) {
             if ((e = (int)hipEventRecord(e3[2], st))) return e;
             ++c->timed;
@@ -838,6 +855,7 @@ int dil_keystore_create(dil_keystore **out, int mode, size_t capacity) {
     k->mode = mode;
     k->capacity = capacity;
     k->pitch = align256(dil_keystore_row_bytes(mode));
+    k->has_sk.assign(capacity, 0);
     const int e = (int)hipMalloc((void **)&k->rows, k->pitch * capacity);
     if (e) {
         delete k;
@@ -858,6 +876,10 @@ size_t dil_keystore_capacity(const dil_keystore *k) { return k ? k->capacity : 0
 
 int dil_keystore_mode(const dil_keystore *k) { return k ? k->mode : 0; }
 
+int dil_keystore_has_secret(const dil_keystore *k, size_t idx) {
+    return k && idx < k->capacity && k->has_sk[idx] ? 1 : 0;
+}
+
 int dil_keystore_load(dil_keystore *k, size_t first, const uint8_t *pk, size_t pk_pitch, const uint8_t *sk,
                       size_t sk_pitch, size_t n, void *stream) {
     if (!k || !pk) return DIL_EINVAL;
@@ -877,6 +899,7 @@ int dil_keystore_load(dil_keystore *k, size_t first, const uint8_t *pk, size_t p
         if (sk && (e = copy2d(row + KY::SK, k->pitch, sk, sk_pitch, Z::SK_BYTES, n,
                                 caller_kind(sk, (n - 1) * sk_pitch + Z::SK_BYTES, hipMemcpyHostToDevice), st)))
             return e;
+        std::fill(k->has_sk.begin() + first, k->has_sk.begin() + first + n, sk ? 1 : 0);
         return launch_key_expand(k->mode, row, k->pitch, n, sk ? 1 : 0, st);
     });
 }
@@ -895,6 +918,7 @@ int dil_keystore_keygen(dil_keystore *k, size_t first, const uint8_t *d_seeds, s
         KeygenArgs a{row + KY::PK, k->pitch, row + KY::SK, k->pitch, row + KY::INTS, k->pitch,
                      d_seeds, seed_pitch, rand_index, n};
         int e = launch_keygen(k->mode, a, st);
+        std::fill(k->has_sk.begin() + first, k->has_sk.begin() + first + n, 1);
         return e ? e : launch_key_expand(k->mode, row, k->pitch, n, 1, st);
     });
 }
@@ -905,6 +929,9 @@ int dil_keystore_export(const dil_keystore *k, size_t first, size_t n, uint8_t *
     if (n == 0) return 0;
     if (first > k->capacity || n > k->capacity - first) return DIL_ERANGE;
     if ((pk && pk_pitch < dil_public_key_bytes(k->mode)) || (sk && sk_pitch < dil_secret_key_bytes(k->mode)))
+        return DIL_EINVAL;
+    // a verify-only key has no secret key to export
+    if (sk && std::find(k->has_sk.begin() + first, k->has_sk.begin() + first + n, 0) != k->has_sk.begin() + first + n)
         return DIL_EINVAL;
     const hipStream_t st = S(stream);
     return with_params(k->mode, [&](auto p) -> int {

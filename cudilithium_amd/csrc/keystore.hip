@@ -55,7 +55,14 @@ __global__ __launch_bounds__(64 * KS_WAVES) void ks_prep_kernel(SignArgs a, int 
         const size_t i = t / NP;
         const int p = (int)(t % NP);
         if (p < NS) {
-            if (with_sk) stage_prep<P>(a, i, p, x[w], zt);
+            if (with_sk) {
+                stage_prep<P>(a, i, p, x[w], zt);
+            } else {
+                // a verify-only key holds no secret: its s1^ / s2^ / t0^ rows are
+                // cleared so that nothing of a row's previous key survives
+                const int32_t zero[4] = {0, 0, 0, 0};
+                wave_store_D(reinterpret_cast<int32_t *>(a.rows + i * a.row_pitch + KY::S1H) + p * N, zero);
+            }
             continue;
         }
         const int k = p - NS;
@@ -86,7 +93,12 @@ __global__ __launch_bounds__(256) void ks_tr_kernel(uint8_t *keys, size_t pitch,
     squeeze_words<4>(s, tr);
 #pragma unroll
     for (int w = 0; w < 4; ++w) reinterpret_cast<uint64_t *>(key + KY::TR)[w] = tr[w];  // 256-B aligned
-    *reinterpret_cast<uint32_t *>(key + KY::FLAGS) = with_sk ? 1u : 0u;
+    *reinterpret_cast<uint32_t *>(key + KY::FLAGS) = with_sk ? KEY_HAS_SECRET : 0u;
+    if (!with_sk) {
+        // verify-only key: clear the secret-key bytes a previous key left in the row
+        uint32_t *skw = reinterpret_cast<uint32_t *>(key + KY::SK);  // 256-B aligned
+        for (int w = 0; w < (Z::SK_BYTES + 3) / 4; ++w) skw[w] = 0;
+    }
 }
 
 template <class P>

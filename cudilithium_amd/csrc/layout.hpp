@@ -97,6 +97,9 @@ struct VerifyLayout {
     }
 };
 
+// FLAGS word of a resident key: set when the key was loaded or generated
+// with its secret key (verify-only keys hold zeros in SK, s1^, s2^, t0^)
+constexpr uint32_t KEY_HAS_SECRET = 1u;
 
 // One resident key of a key store: public key, secret key (absent for
 // verify-only keys), tr = H(pk), and the expanded arrays every signature and
@@ -109,7 +112,7 @@ struct KeyLayout {
     static constexpr size_t PK = 0;
     static constexpr size_t SK = align256(Z::PK_BYTES);
     static constexpr size_t TR = SK + align256(Z::SK_BYTES);  // tr (32) | flags (4)
-    static constexpr size_t FLAGS = TR + SEEDBYTES;
+    static constexpr size_t FLAGS = TR + SEEDBYTES;           // KEY_HAS_SECRET or 0 (verify-only)
     static constexpr size_t INTS = TR + 256;
     static constexpr size_t MAT = INTS;
     static constexpr size_t S1H = MAT + (size_t)P::K * P::L * N * 4;

@@ -76,6 +76,15 @@ int launch_sign_round(int mode, const SignArgs &a, const SlotRef &sr, const uint
                       int by_item, uint8_t *d_done, hipStream_t st);
 int launch_sign_copy(int mode, const SignArgs &a, const SlotRef &sr, const uint64_t *d_copy, size_t ncopy,
                      hipStream_t st);
+// Status bits of a signing call (the control word after the queue head).
+constexpr uint32_t SIGN_STATUS_ROUND_CAP = 1u;  // a workgroup stopped at its round limit
+constexpr uint32_t SIGN_STATUS_NO_SECRET = 2u;  // an item's resident key has no secret key
+// Resident keys: zeroes the signatures of items whose key has no secret key
+// and sets SIGN_STATUS_NO_SECRET in *status (no-op without key indices).
+int launch_sign_key_check(int mode, const SignArgs &a, uint32_t *status, hipStream_t st);
+// Clears the control words d_head[0..2] (a kernel, so graph capture records
+// no memset node).
+int launch_zero_ctl(uint32_t *d_head, hipStream_t st);
 int sign_perstage(int mode, const SignArgs &a, const SlotRef &item_slots, const SlotRef &temp_slots, size_t thr,
                   SignLutEntry *sign_lut, uint32_t *h_exec, uint32_t *d_exec, uint8_t *h_done, uint8_t *d_done,
                   uint64_t *h_copy, uint64_t *d_copy, hipStream_t st);

@@ -471,6 +471,7 @@ int dil_server_submit(dil_server *s, dil_request *const *reqs, size_t n) {
             if (!bad && r->op == DIL_OP_KEYGEN && (!r->seed || !r->out)) bad = DIL_EINVAL;
             if (!bad && r->op != DIL_OP_KEYGEN) {
                 if (!s->ks[ms] || r->key >= dil_keystore_capacity(s->ks[ms])) bad = DIL_EINVAL;
+                else if (r->op == DIL_OP_SIGN && !dil_keystore_has_secret(s->ks[ms], r->key)) bad = DIL_EINVAL;
                 else if (r->mlen && !r->msg) bad = DIL_EINVAL;
                 else if (al8(r->mlen) > s->msg_cap) bad = DIL_ERANGE;
                 else if (r->op == DIL_OP_SIGN ? !r->out : !r->sig) bad = DIL_EINVAL;

@@ -118,6 +118,29 @@ __global__ __launch_bounds__(256) void sg_copy_kernel(SignArgs a, SlotRef sr, co
     }
 }
 
+// Resident-key signing: an item whose key was loaded without its secret key
+// (FLAGS lacks KEY_HAS_SECRET; its secret arrays are zero) gets an all-zero
+// signature and sets SIGN_STATUS_NO_SECRET in *status.  Runs after the
+// rejection loop, lane per item.
+template <class P>
+__global__ __launch_bounds__(256) void sg_key_check_kernel(SignArgs a, uint32_t *status) {
+    using KY = KeyLayout<P>;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const uint8_t *key_row = a.key_ints - KY::MAT + sk_index(a, i) * a.key_pitch;
+    if (*reinterpret_cast<const uint32_t *>(key_row + KY::FLAGS) & KEY_HAS_SECRET) return;
+    uint8_t *sig = a.sig + i * a.sig_pitch;
+    for (int b = 0; b < Sizes<P>::SIG_BYTES; ++b) sig[b] = 0;
+    atomicOr(status, SIGN_STATUS_NO_SECRET);
+}
+
+template <class P>
+int sign_key_check_impl(const SignArgs &a, uint32_t *status, hipStream_t st) {
+    if (a.n == 0 || !a.key_idx) return 0;
+    hipLaunchKernelGGL(sg_key_check_kernel<P>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a, status);
+    return (int)hipGetLastError();
+}
+
 // The three fixed-stage launches are independent: mu/rhoprime (one lane per
 // item, latency-bound on long messages), ExpandA (one lane per matrix entry,
 // throughput-bound) and the s1/s2/t0 NTTs run as a fork/join over the
@@ -181,6 +204,10 @@ int sign_copy_impl(const SignArgs &a, const SlotRef &sr, const uint64_t *d_copy,
     }
 
 int launch_sign_fixed(int mode, const SignArgs &a, hipStream_t st) { DIL_DISPATCH(mode, sign_fixed_impl, a, st) }
+
+int launch_sign_key_check(int mode, const SignArgs &a, uint32_t *status, hipStream_t st) {
+    DIL_DISPATCH(mode, sign_key_check_impl, a, status, st)
+}
 
 int launch_sign_round(int mode, const SignArgs &a, const SlotRef &sr, const uint32_t *d_exec, size_t nslot,
                       int by_item, uint8_t *d_done, hipStream_t st) {

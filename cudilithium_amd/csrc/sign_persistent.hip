@@ -12,6 +12,11 @@ __global__ void zero_ctl_kernel(uint32_t *p, unsigned long long *phase_sums) {
         for (int w = 0; w < PERSIST_PHASES; ++w) phase_sums[w] = 0;
 }
 
+int launch_zero_ctl(uint32_t *d_head, hipStream_t st) {
+    hipLaunchKernelGGL(zero_ctl_kernel, dim3(1), dim3(1), 0, st, d_head, nullptr);
+    return (int)hipGetLastError();
+}
+
 int launch_sign_persistent(int mode, const SignArgs &a, const SlotRef &sr, size_t nslots, int slots_per_block,
                            int occupancy, int blocks, int claim, uint32_t *d_head, unsigned long long *phase_sums,
                            hipStream_t st) {

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
         if (round == PERSIST_MAX_ROUNDS) {
             // no correct run gets here (each round succeeds per item with
             // probability >= 1/5); stop instead of spinning and report it
-            if (tid == 0) atomicOr(head + 1, 1u);
+            if (tid == 0) atomicOr(head + 1, SIGN_STATUS_ROUND_CAP);
             break;
         }
         if (tid == 0) {

@@ -228,9 +228,12 @@ int dil_sign_device(dil_sign_ctx *ctx, uint8_t *d_sig, size_t sig_pitch, const u
  * sequential specification makes (winning nonce + 1), copied to host and
  * synchronised on `stream`. */
 int dil_sign_ctx_attempt_counts(dil_sign_ctx *ctx, uint32_t *h_counts, size_t n, void *stream);
-/* Status of the last persistent-engine call (synchronises `stream`): 0, or
- * non-zero if a workgroup stopped at its round limit (never expected; the
- * batch's signatures are then incomplete). */
+/* Status of the last call (synchronises `stream`): 0, or a combination of
+ * DIL_SIGN_STATUS_ROUND_CAP (a persistent workgroup stopped at its round
+ * limit; never expected; the batch's signatures are then incomplete) and
+ * DIL_SIGN_STATUS_NO_SECRET (resident signing: an item's key is verify-only). */
+#define DIL_SIGN_STATUS_ROUND_CAP 1
+#define DIL_SIGN_STATUS_NO_SECRET 2
 int dil_sign_ctx_status(dil_sign_ctx *ctx, int *status, void *stream);
 /* Attempts the last persistent-engine call executed, speculative ones
  * included (synchronises `stream`). */
@@ -255,8 +258,12 @@ int dil_keystore_create(dil_keystore **ks, int mode, size_t capacity);
 int dil_keystore_destroy(dil_keystore *ks);
 size_t dil_keystore_capacity(const dil_keystore *ks);
 int dil_keystore_mode(const dil_keystore *ks);
+/* 1 if key idx was loaded with its secret key or generated in the store, 0
+ * for verify-only keys and keys never installed. */
+int dil_keystore_has_secret(const dil_keystore *ks, size_t idx);
 /* Installs n keys at [first, first + n) from pk (and sk, or NULL for
- * verify-only keys), host or device memory, and expands them on `stream`. */
+ * verify-only keys), host or device memory, and expands them on `stream`.
+ * A verify-only load clears the rows' secret-key bytes and s1^, s2^, t0^. */
 int dil_keystore_load(dil_keystore *ks, size_t first, const uint8_t *pk, size_t pk_pitch,
                       const uint8_t *sk, size_t sk_pitch, size_t n, void *stream);
 /* Generates n keys into [first, first + n) as dil_keypair_device would
@@ -264,15 +271,18 @@ int dil_keystore_load(dil_keystore *ks, size_t first, const uint8_t *pk, size_t 
  * and expands them. */
 int dil_keystore_keygen(dil_keystore *ks, size_t first, const uint8_t *d_seeds, size_t seed_pitch,
                         uint64_t rand_index, size_t n, void *stream);
-/* Copies pk and/or sk (either may be NULL) of keys [first, first + n) out. */
+/* Copies pk and/or sk (either may be NULL) of keys [first, first + n) out;
+ * DIL_EINVAL when sk is asked for and a key in the range is verify-only. */
 int dil_keystore_export(const dil_keystore *ks, size_t first, size_t n, uint8_t *pk, size_t pk_pitch,
                         uint8_t *sk, size_t sk_pitch, void *stream);
 /* A signing context whose rows omit the per-item expanded arrays; it only
  * accepts dil_sign_device_resident. */
 int dil_sign_ctx_create_resident(dil_sign_ctx **ctx, int mode, size_t max_batch, size_t nslots);
 /* dil_sign_device with item i signed by resident key d_key_idx[i] (device
- * array; keys must have been loaded with their secret key).  Indices must
- * be below the store's capacity (larger ones are clamped to the last key). */
+ * array).  Indices must be below the store's capacity (larger ones are
+ * clamped to the last key).  An item whose key is verify-only gets an
+ * all-zero signature, and the call's status (dil_sign_ctx_status) has
+ * DIL_SIGN_STATUS_NO_SECRET set. */
 int dil_sign_device_resident(dil_sign_ctx *ctx, const dil_keystore *ks, const uint32_t *d_key_idx,
                              uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_msg,
                              const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, int engine,

@@ -44,9 +44,24 @@ def test_perstage_engine_matches_oracle(mode):
     assert np.array_equal(got, exp)
 
 
-def test_config1_dil2_batch128_explicit_seeds():
+def test_persistent_dropin_fewer_slots_than_items():
+    """Drop-in sign with exec_threshold 8 on 300 Dilithium3 items: the
+    persistent engine gets exec_threshold * temp_pitch / align256(sig) (79)
+    slots, keeping slot state in sign-pool rows; bytes equal the oracle's."""
+    from cudilithium_amd.api import Dilithium, ENGINE_PERSISTENT
+    mode, n = 3, 300
+    pk, sk = oracle.keypair_counter_batch(mode, n, rand_index=4242)
+    m = np.random.default_rng(4242).integers(0, 256, (n, 33), dtype=np.uint8)
+    exp, _ = oracle.sign_batch(mode, m, sk)
+    got = Dilithium(mode).sign(m, sk, exec_threshold=8, engine=ENGINE_PERSISTENT)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("engine", _engines())
+def test_config1_dil2_batch128_explicit_seeds(engine):
     """Dilithium2, batch 128, 32-byte seeds and 64-byte messages from the
-    seeded xoshiro256** stream: GPU bytes equal the oracle's, verdicts 0."""
+    seeded xoshiro256** stream: GPU bytes equal the oracle's, verdicts 0, for
+    the persistent engine and the single-launch-per-stage form."""
     from cudilithium_amd.api import Dilithium, keypair_device
     from cudilithium_amd.workloads import config1
     seeds, m = config1()
@@ -54,7 +69,7 @@ def test_config1_dil2_batch128_explicit_seeds():
     epk, esk = oracle.keypair_batch(2, seeds)
     assert np.array_equal(pk, epk) and np.array_equal(sk, esk)
     d = Dilithium(2)
-    sig = d.sign(m, sk)
+    sig = d.sign(m, sk, engine=engine)
     esig, _ = oracle.sign_batch(2, m, sk)
     assert np.array_equal(sig, esig)
     assert (d.verify(sig, m, pk) == 0).all()
@@ -210,3 +225,28 @@ def test_persistent_engine_claim_cap(claim):
     assert np.array_equal(got, exp)
     assert np.array_equal(ctx.attempt_counts(n), exp_att)
     assert sum(ctx.phase_cycles().values()) > 0
+
+
+def test_verify_rejects_hint_padding_pair_like_specification():
+    """Two non-zero hint padding bytes 32 positions apart: the reference's
+    XOR-of-ballots decode (src/verify.cu:81-85) would accept; the engine
+    follows the specification and rejects, as the oracle does."""
+    from cudilithium_amd.api import Dilithium
+    mode, n = 5, 128
+    p = oracle.params(mode)
+    pk, sk = oracle.keypair_counter_batch(mode, n, rand_index=31)
+    msgs = np.array([[i] * 40 for i in range(n)], dtype=np.uint8)
+    sig, _ = oracle.sign_batch(mode, msgs, sk)
+    hoff = p.sig_bytes - (p.omega + p.k)
+    cand = [i for i in range(n) if int(sig[i, hoff + p.omega + p.k - 1]) + 32 < p.omega]
+    assert cand
+    bad = sig[cand].copy()
+    for j, i in enumerate(cand):
+        pos = int(sig[i, hoff + p.omega + p.k - 1])
+        bad[j, hoff + pos] = 1
+        bad[j, hoff + pos + 32] = 1
+    d = Dilithium(mode)
+    assert (d.verify(sig[cand], msgs[cand], pk[cand]) == 0).all()
+    got = d.verify(bad, msgs[cand], pk[cand])
+    assert (got == -1).all()
+    assert np.array_equal(got, oracle.verify_batch(mode, bad, msgs[cand], pk[cand]))

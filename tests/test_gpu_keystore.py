@@ -70,6 +70,53 @@ def test_keystore_load_host_keys_and_verify_only_keys():
     assert (vonly.verify(bad_idx, exp, msgs) == -1).all()
 
 
+def test_verify_only_load_clears_secrets_and_signing_reports_it():
+    """A verify-only load over rows that held signing keys leaves no secret
+    behind: export of sk is refused, signing under such a key gives an
+    all-zero signature with DIL_SIGN_STATUS_NO_SECRET (both engines), and the
+    other items of the batch are still the oracle's signatures."""
+    from cudilithium_amd import api, _lib
+    mode, nkeys, n = 3, 6, 30
+    pk, sk = oracle.keypair_counter_batch(mode, nkeys, rand_index=1200)
+    ks = api.KeyStore(mode, nkeys)
+    ks.load(0, pk, sk)
+    assert all(ks.has_secret(i) for i in range(nkeys))
+    other_pk, _ = oracle.keypair_counter_batch(mode, 2, rand_index=1300)
+    ks.load(2, other_pk)  # keys 2 and 3 become verify-only
+    assert [ks.has_secret(i) for i in range(nkeys)] == [True, True, False, False, True, True]
+    with pytest.raises(_lib.DilError):
+        ks.export(0, nkeys)
+    pk_out, _ = ks.export(0, nkeys, with_sk=False)
+    assert np.array_equal(pk_out[2:4], other_pk) and np.array_equal(pk_out[4:], pk[4:])
+    assert np.array_equal(ks.export(4, 2)[1], sk[4:])
+
+    key_idx = (np.arange(n) % nkeys).astype(np.uint32)
+    msgs = _msgs(1400, n, 1, 500)
+    good = np.nonzero(key_idx != 2)[0]
+    good = good[key_idx[good] != 3]
+    exp, _ = oracle.sign_batch(mode, [msgs[i] for i in good], sk[key_idx[good]])
+    for engine in (api.ENGINE_PERSISTENT, api.ENGINE_PERSTAGE):
+        ctx = api.SignContext(mode, n, n, resident_only=True)
+        d_idx = api.DeviceBuffer.from_array(key_idx)
+        dm = api.DeviceMessages(msgs)
+        d_sig = api.DeviceBuffer(n * ctx.sig_bytes)
+        d_sig.memset(0xAB)
+        ctx.sign_resident_device(ks, d_idx, d_sig, dm, engine)
+        assert ctx.status() == api.STATUS_NO_SECRET, engine
+        got = d_sig.download((n, ctx.sig_bytes), np.uint8)
+        bad = np.isin(key_idx, [2, 3])
+        assert not got[bad].any(), engine
+        assert np.array_equal(got[good], exp), engine
+        with pytest.raises(_lib.DilError):
+            ctx.sign_resident(ks, key_idx, msgs, engine)
+        # a batch of signing keys only clears the status again
+        assert np.array_equal(ctx.sign_resident(ks, key_idx[good], [msgs[i] for i in good], engine), exp)
+        assert ctx.status() == 0
+    # verification under the verify-only keys still works
+    vsig, _ = oracle.sign_batch(mode, [b"m2", b"m3"], oracle.keypair_counter_batch(mode, 2, rand_index=1300)[1])
+    assert (ks.verify(np.array([2, 3], np.uint32), vsig, [b"m2", b"m3"]) == 0).all()
+
+
 def test_resident_only_context_rejects_plain_sign():
     from cudilithium_amd import api, _lib
     mode = 2

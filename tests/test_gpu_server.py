@@ -143,6 +143,21 @@ def test_server_admission_errors():
     assert batch.reqs[2].verdict == -1
     exp, _ = oracle.sign_batch(2, [b"ok"], keys[2][1][1:2])
     assert np.array_equal(batch.out[3], exp[0])
+
+    # a sign request under a verify-only key is refused at admission; verify
+    # under that key is served
+    vpk, vsk = oracle.keypair_counter_batch(5, 1, rand_index=77)
+    srv.stores[5].load(KEYS - 1, vpk)
+    vsig, _ = oracle.sign_batch(5, [b"hello"], vsk)
+    batch = api.RequestBatch(np.array([api.OP_SIGN, api.OP_VERIFY, api.OP_SIGN]), np.array([5, 5, 5]),
+                             np.array([KEYS - 1, KEYS - 1, 0]), [b"hello", b"hello", b"hello"],
+                             [None, vsig[0], None], np.zeros((3, 32), np.uint8))
+    srv.submit(batch.reqs)
+    srv.drain()
+    assert [r.status for r in batch.reqs] == [-1, 0, 0]
+    assert batch.reqs[1].verdict == 0
+    exp, _ = oracle.sign_batch(5, [b"hello"], keys[5][1][0:1])
+    assert np.array_equal(batch.out[2], exp[0])
     srv.close()
 
 
