@@ -84,6 +84,7 @@ def _register():
     R("dil_keystore_capacity", c_size_t, [c_void_p])
     R("dil_keystore_mode", c_int, [c_void_p])
     R("dil_keystore_has_secret", c_int, [c_void_p, c_size_t])
+    R("dil_release_host_staging", c_int, [])
     R("dil_keystore_load", c_int, [c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t, c_void_p])
     R("dil_keystore_keygen", c_int, [c_void_p, c_size_t, c_void_p, c_size_t, ctypes.c_uint64, c_size_t, c_void_p])
     R("dil_keystore_export", c_int, [c_void_p, c_size_t, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p])
@@ -146,6 +147,34 @@ class TaskLut:
         self.d_copy = DeviceBuffer(16 * exec_threshold)
         self.c = TaskLutC(ctypes.cast(self.sign_lut, ctypes.POINTER(SignLutElement)), batch, self.h_exec.ptr,
                           self.h_done.ptr, self.h_copy.ptr, self.d_exec.ptr, self.d_done.ptr, self.d_copy.ptr)
+
+
+class TaskLutSlice:
+    """Entries [first, first + count) of a TaskLut's exec / done / copy tables
+    with a sign_lut of its own, as the reference's stream mode hands each
+    stream a slice (benches/bench_dilithium.cu:205-217)."""
+
+    def __init__(self, lut: TaskLut, first: int, count: int, batch: int):
+        assert 0 < count and first + count <= lut.exec_threshold
+        self.exec_threshold = count
+        self.sign_lut = (SignLutElement * max(1, batch))()
+        self.c = TaskLutC(ctypes.cast(self.sign_lut, ctypes.POINTER(SignLutElement)), batch,
+                          lut.h_exec.ptr + 4 * first, lut.h_done.ptr + first, lut.h_copy.ptr + 16 * first,
+                          lut.d_exec.ptr + 4 * first, lut.d_done.ptr + first, lut.d_copy.ptr + 16 * first)
+
+
+def crypto_sign_signature_rows(mode, sig_ptr: int, sig_pitch: int, m_ptr: int | None, m_pitch: int, mlen: int,
+                               sk_ptr: int, sign_pool_ptr: int, sign_pitch: int, temp_pool_ptr: int,
+                               temp_pitch: int, lut, exec_threshold: int, batch_size: int, stream=None,
+                               engine: int = ENGINE_PERSISTENT) -> int:
+    """crypto_sign_signature on raw addresses (host rows and pool rows of a
+    slice); `lut` is a TaskLut or TaskLutSlice.  Returns siglen."""
+    fn = L().dil_crypto_sign_signature if engine == ENGINE_PERSISTENT else L().dil_crypto_sign_signature_perstage
+    siglen = c_size_t(0)
+    r = fn(mode, sig_ptr, sig_pitch, ctypes.byref(siglen), m_ptr, m_pitch, mlen, sk_ptr, sign_pool_ptr, sign_pitch,
+           temp_pool_ptr, temp_pitch, ctypes.byref(lut.c), exec_threshold, batch_size, stream)
+    _lib.check(r, "crypto_sign_signature")
+    return siglen.value
 
 
 def crypto_sign_keypair(mode, pk: np.ndarray, sk: np.ndarray, d_pool: PitchedBuffer, pitch: int,

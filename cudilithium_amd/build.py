@@ -70,22 +70,25 @@ def build(verbose: bool = True) -> str:
 
 
 def build_cxx_tests() -> None:
-    """The C++ drop-in KAT program (tests/cxx/kat_dropin.cpp), one binary per
-    mode, linked against the in-tree library."""
+    """The C++ drop-in test programs (tests/cxx/*.cpp), one binary per mode,
+    linked against the in-tree library: the KAT flow, and randomized signing
+    built with -DDILITHIUM_RANDOMIZED_SIGNING."""
     root = os.path.dirname(HERE)
-    src = os.path.join(root, "tests", "cxx", "kat_dropin.cpp")
     out_dir = os.path.join(root, "tests", "cxx", "bin")
-    if not os.path.exists(src):
-        return
-    os.makedirs(out_dir, exist_ok=True)
-    for mode in (2, 3, 5):
-        out = os.path.join(out_dir, f"kat_dropin_{mode}")
-        cmd = [HIPCC, "-std=c++17", "-O2", f"-DDILITHIUM_MODE={mode}", "-Wno-unused-value", "-Wno-unused-result",
-               f"-I{os.path.join(root, 'include')}", src, f"-L{HERE}", "-lcudilithium_amd",
-               "-Wl,-rpath,$ORIGIN/../../../cudilithium_amd", "-o", out]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError(f"building {out} failed:\n{r.stderr}")
+    progs = (("kat_dropin", []), ("rnd_dropin", ["-DDILITHIUM_RANDOMIZED_SIGNING"]))
+    for name, extra in progs:
+        src = os.path.join(root, "tests", "cxx", f"{name}.cpp")
+        if not os.path.exists(src):
+            continue
+        os.makedirs(out_dir, exist_ok=True)
+        for mode in (2, 3, 5):
+            out = os.path.join(out_dir, f"{name}_{mode}")
+            cmd = [HIPCC, "-std=c++17", "-O2", f"-DDILITHIUM_MODE={mode}", *extra, "-Wno-unused-value",
+                   "-Wno-unused-result", f"-I{os.path.join(root, 'include')}", src, f"-L{HERE}",
+                   "-lcudilithium_amd", "-Wl,-rpath,$ORIGIN/../../../cudilithium_amd", "-o", out]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"building {out} failed:\n{r.stderr}")
 
 
 if __name__ == "__main__":

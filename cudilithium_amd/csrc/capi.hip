@@ -4,7 +4,10 @@
 
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
+#include <utility>
+#include <vector>
 
 #include "../../include/cudilithium_amd.h"
 #include "internal.hpp"
@@ -24,23 +27,32 @@ int device_cu_count() {
     return cached[dev];
 }
 
-AuxStreams *aux_streams() {
-    static AuxStreams *per_dev[64] = {nullptr};
+AuxStreams *aux_streams(hipStream_t caller) {
+    // per device: one set per caller stream (up to kAuxSets, then shared by
+    // hash), so that callers on different streams fork onto different
+    // auxiliary streams and their fixed stages run concurrently on the GPU
+    constexpr size_t kAuxSets = 16;
+    struct PerDevice {
+        std::vector<std::pair<hipStream_t, AuxStreams *>> sets;
+    };
+    static PerDevice per_dev[64];
     static std::mutex mu;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
     std::lock_guard<std::mutex> g(mu);
-    if (!per_dev[dev]) {
-        AuxStreams *a = new AuxStreams();
-        bool ok = hipStreamCreateWithFlags(&a->s[0], hipStreamNonBlocking) == hipSuccess &&
-                  hipStreamCreateWithFlags(&a->s[1], hipStreamNonBlocking) == hipSuccess &&
-                  hipEventCreateWithFlags(&a->fork, hipEventDisableTiming) == hipSuccess &&
-                  hipEventCreateWithFlags(&a->join[0], hipEventDisableTiming) == hipSuccess &&
-                  hipEventCreateWithFlags(&a->join[1], hipEventDisableTiming) == hipSuccess;
-        if (!ok) return nullptr;
-        per_dev[dev] = a;
-    }
-    return per_dev[dev];
+    auto &v = per_dev[dev].sets;
+    for (auto &e : v)
+        if (e.first == caller) return e.second;
+    if (v.size() >= kAuxSets) return v[std::hash<void *>()(reinterpret_cast<void *>(caller)) % v.size()].second;
+    AuxStreams *a = new AuxStreams();
+    bool ok = hipStreamCreateWithFlags(&a->s[0], hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&a->s[1], hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&a->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&a->join[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&a->join[1], hipEventDisableTiming) == hipSuccess;
+    if (!ok) return nullptr;  // (a partly created set is not reused)
+    v.emplace_back(caller, a);
+    return a;
 }
 
 int default_grid_blocks(size_t units, int per_block, int blocks_per_cu) {

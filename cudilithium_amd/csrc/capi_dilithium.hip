@@ -2,8 +2,11 @@
 // replacements of the reference's api.cuh surface and the device-resident
 // extensions.
 #include <hip/hip_runtime.h>
+#include <sys/random.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cerrno>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -229,6 +232,19 @@ static_assert(sizeof(dil_copy_lut_element) == 2 * sizeof(uint64_t), "copy_lut_el
 
 extern "C" {
 
+/* ------------------------------------------------------- staging buffer */
+
+int dil_release_host_staging(void) {
+    // waits for a copy that holds the buffer; the next pageable copy
+    // allocates it again
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    if (!g_stage) return 0;
+    const int e = (int)hipHostFree(g_stage);
+    g_stage = nullptr;
+    g_stage_dev = nullptr;
+    return e;
+}
+
 /* ------------------------------------------------------------ pool sizes */
 
 size_t dil_keypair_pool_row_bytes(int mode) {
@@ -291,10 +307,11 @@ int dil_crypto_sign_keypair(int mode, uint8_t *pk, uint8_t *sk, uint8_t *d_pool,
 
 /* --------------------------------------------------------- drop-in: sign */
 
+// rnd (randomized signing): batch x 64 host bytes of rhoprime, or null
 static int sign_dropin(int mode, int engine, uint8_t *sig, size_t sig_pitch, size_t *siglen, const uint8_t *m,
                        size_t m_pitch, size_t mlen, const uint8_t *sk, uint8_t *d_sign_pool, size_t sign_pitch,
                        uint8_t *d_temp_pool, size_t temp_pitch, dil_task_lut *lut, size_t exec_threshold,
-                       size_t batch, void *stream) {
+                       size_t batch, void *stream, const uint8_t *rnd = nullptr) {
     if (!valid_mode(mode)) return DIL_EINVAL;
     if (siglen) *siglen = dil_signature_bytes(mode);
     if (batch == 0) return 0;
@@ -320,6 +337,11 @@ static int sign_dropin(int mode, int engine, uint8_t *sig, size_t sig_pitch, siz
         if ((e = copy2d(d_sign_pool + SLy::SK, sign_pitch, sk, Z::SK_BYTES, Z::SK_BYTES, batch, hipMemcpyHostToDevice,
                         st)))
             return e;
+        // randomized signing: rhoprime rows into each row's rhoprime field,
+        // which the fixed stage then reads instead of deriving it (api.cu:143-148)
+        if (rnd && (e = copy2d(d_sign_pool + SLy::RHOPRIME, sign_pitch, rnd, CRHBYTES, CRHBYTES, batch,
+                               hipMemcpyHostToDevice, st)))
+            return e;
         SignArgs a{};
         a.sig = d_sign_pool + SLy::SIG;
         a.sig_pitch = sign_pitch;
@@ -330,6 +352,10 @@ static int sign_dropin(int mode, int engine, uint8_t *sig, size_t sig_pitch, siz
         a.row_pitch = sign_pitch;
         a.off_ints = SLy::ints(mlen);
         a.n = batch;
+        if (rnd) {
+            a.rnd = d_sign_pool + SLy::RHOPRIME;
+            a.rnd_pitch = sign_pitch;
+        }
         const SlotRef item_slots{d_sign_pool + SLy::SLOT, sign_pitch, d_sign_pool + SLy::SIG, sign_pitch};
         const SlotRef temp_slots{d_temp_pool + TempLayout<P>::SLOT, temp_pitch, d_temp_pool + TempLayout<P>::SIG,
                                  temp_pitch};
@@ -386,6 +412,56 @@ int dil_crypto_sign_signature_perstage(int mode, uint8_t *sig, size_t sig_pitch,
                                        dil_task_lut *lut, size_t exec_threshold, size_t batch, void *stream) {
     return sign_dropin(mode, DIL_ENGINE_PERSTAGE, sig, sig_pitch, siglen, m, m_pitch, mlen, sk, d_sign_pool,
                        sign_pitch, d_temp_pool, temp_pitch, lut, exec_threshold, batch, stream);
+}
+
+namespace {
+
+int os_randombytes(uint8_t *out, size_t outlen) {
+    while (outlen) {
+        const ssize_t r = getrandom(out, outlen, 0);
+        if (r < 0) {
+            if (errno == EINTR) continue;
+            return DIL_EINVAL;
+        }
+        out += r;
+        outlen -= (size_t)r;
+    }
+    return 0;
+}
+
+std::atomic<dil_randombytes_fn> g_randombytes{nullptr};
+
+}  // namespace
+
+void dil_set_randombytes(dil_randombytes_fn fn) { g_randombytes.store(fn); }
+
+int dil_randombytes(uint8_t *out, size_t outlen) {
+    if (!out && outlen) return DIL_EINVAL;
+    const dil_randombytes_fn fn = g_randombytes.load();
+    return fn ? fn(out, outlen) : os_randombytes(out, outlen);
+}
+
+int dil_crypto_sign_signature_randomized(int mode, uint8_t *sig, size_t sig_pitch, size_t *siglen, const uint8_t *m,
+                                         size_t m_pitch, size_t mlen, const uint8_t *sk, uint8_t *d_sign_pool,
+                                         size_t sign_pitch, uint8_t *d_temp_pool, size_t temp_pitch,
+                                         dil_task_lut *lut, size_t exec_threshold, size_t batch, void *stream) {
+    if (!valid_mode(mode)) return DIL_EINVAL;
+    if (batch == 0) {
+        if (siglen) *siglen = dil_signature_bytes(mode);
+        return 0;
+    }
+    if (batch > 65536) return DIL_ERANGE;
+    std::vector<uint8_t> rnd;
+    try {
+        rnd.resize(batch * CRHBYTES);
+    } catch (...) {
+        return DIL_ENOMEM;
+    }
+    const int e = dil_randombytes(rnd.data(), rnd.size());
+    if (e) return e;
+    // the staged copy of the pageable rows completes before sign_dropin returns
+    return sign_dropin(mode, DIL_ENGINE_PERSISTENT, sig, sig_pitch, siglen, m, m_pitch, mlen, sk, d_sign_pool,
+                       sign_pitch, d_temp_pool, temp_pitch, lut, exec_threshold, batch, stream, rnd.data());
 }
 
 /* ------------------------------------------------------- drop-in: verify */
@@ -473,6 +549,31 @@ int dil_keypair_device(int mode, uint8_t *d_pk, size_t pk_pitch, uint8_t *d_sk, 
 }
 
 size_t dil_verify_work_row_bytes(int mode) { return dil_verify_pool_row_bytes(mode, 0); }
+
+int dil_verify_device_timed(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_pk,
+                            size_t pk_pitch, const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen,
+                            size_t n, uint8_t *d_work, size_t work_pitch, float *kernel_ms3, void *stream) {
+    if (!kernel_ms3) return DIL_EINVAL;
+    if (!valid_mode(mode)) return DIL_EINVAL;
+    if (n == 0) return 0;
+    if (!d_ret || !d_sig || !d_pk || !d_work || !d_moff || !d_mlen || !d_msg) return DIL_EINVAL;
+    if (sig_pitch < dil_signature_bytes(mode) || pk_pitch < dil_public_key_bytes(mode)) return DIL_EINVAL;
+    if (work_pitch < dil_verify_work_row_bytes(mode) || (work_pitch & 255)) return DIL_ENOMEM;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int e = 0;
+    for (auto &x : ev)
+        if (!e) e = (int)hipEventCreate(&x);
+    if (!e) {
+        VerifyArgs a{d_sig, sig_pitch, d_pk, pk_pitch, MsgRef{d_msg, d_moff, 0, d_mlen, 0}, d_work, work_pitch, d_ret,
+                     1, n};
+        e = launch_verify(mode, a, S(stream), ev);
+        if (!e) e = (int)hipEventSynchronize(ev[3]);
+        for (int k = 0; k < 3 && !e; ++k) e = (int)hipEventElapsedTime(&kernel_ms3[k], ev[k], ev[k + 1]);
+    }
+    for (auto &x : ev)
+        if (x) (void)hipEventDestroy(x);
+    return e;
+}
 
 int dil_verify_device(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_pk,
                       size_t pk_pitch, const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen,

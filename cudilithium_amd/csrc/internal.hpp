@@ -21,15 +21,18 @@ int device_cu_count();
 int default_grid_blocks(size_t units, int per_block, int blocks_per_cu = 8);
 
 // Two auxiliary streams (default priority, non-blocking) and fork/join
-// events of the current device, created on first use and kept until the
-// process ends.  A caller holds `lock` while it enqueues a fork/join so that
-// host threads sharing the device cannot interleave their event records.
+// events of the current device for work forked from the caller's stream
+// `caller`, created on first use and kept until the process ends.  Each
+// caller stream gets a set of its own (up to a cap, then sets are shared),
+// so forks from different streams do not queue behind each other.  A caller
+// holds `lock` while it enqueues a fork/join so that host threads sharing a
+// set cannot interleave their event records.
 struct AuxStreams {
     hipStream_t s[2];
     hipEvent_t fork, join[2];
     std::mutex lock;
 };
-AuxStreams *aux_streams();
+AuxStreams *aux_streams(hipStream_t caller);
 
 // Launchers (return a hipError_t value, 0 on success).
 int launch_ntt(int kind, int32_t *c, const int32_t *a, const int32_t *b, size_t npoly, size_t pitch_words,

@@ -70,7 +70,8 @@ int launch_keygen(int mode, const KeygenArgs &a, hipStream_t st);
 // Expands keys [0, n) of a key store (KeyLayout rows from `keys`): tr, A and
 // t1^ from the public key, and s1^, s2^, t0^ when with_sk.
 int launch_key_expand(int mode, uint8_t *keys, size_t key_pitch, size_t n, int with_sk, hipStream_t st);
-int launch_verify(int mode, const VerifyArgs &a, hipStream_t st);
+// ev: null, or four events recorded around vf_prep / vf_main / vf_final
+int launch_verify(int mode, const VerifyArgs &a, hipStream_t st, hipEvent_t *ev = nullptr);
 int launch_sign_fixed(int mode, const SignArgs &a, hipStream_t st);
 int launch_sign_round(int mode, const SignArgs &a, const SlotRef &sr, const uint32_t *d_exec, size_t nslot,
                       int by_item, uint8_t *d_done, hipStream_t st);

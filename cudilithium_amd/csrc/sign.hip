@@ -156,7 +156,7 @@ int sign_fixed_impl(const SignArgs &a, hipStream_t st) {
         hipLaunchKernelGGL(sg_mu_kernel<P>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a);
         return (int)hipGetLastError();
     }
-    AuxStreams *aux = aux_streams();
+    AuxStreams *aux = aux_streams(st);
     if (!aux) return (int)hipErrorOutOfMemory;
     std::lock_guard<std::mutex> guard(aux->lock);
     int e;

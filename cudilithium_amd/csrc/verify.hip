@@ -269,22 +269,29 @@ static int launch_vf_main(const VerifyArgs &a, hipStream_t st) {
     return 0;
 }
 
+// ev (optional): four events recorded before vf_prep, between the launches
+// and after vf_final, for per-kernel timing on the launch stream
 template <class P>
-static int verify_impl(const VerifyArgs &a, hipStream_t st) {
+static int verify_impl(const VerifyArgs &a, hipStream_t st, hipEvent_t *ev) {
     if (a.n == 0) return 0;
     const unsigned b1 = (unsigned)((a.n + 255) / 256);
+    int e;
+    if (ev && (e = (int)hipEventRecord(ev[0], st))) return e;
     hipLaunchKernelGGL(vf_prep_kernel<P>, dim3(b1), dim3(256), 0, st, a);
-    const int e = a.key_idx ? launch_vf_main<P, true>(a, st) : launch_vf_main<P, false>(a, st);
+    if (ev && (e = (int)hipEventRecord(ev[1], st))) return e;
+    e = a.key_idx ? launch_vf_main<P, true>(a, st) : launch_vf_main<P, false>(a, st);
     if (e) return e;
+    if (ev && (e = (int)hipEventRecord(ev[2], st))) return e;
     hipLaunchKernelGGL(vf_final_kernel<P>, dim3(b1), dim3(256), 0, st, a);
+    if (ev && (e = (int)hipEventRecord(ev[3], st))) return e;
     return (int)hipGetLastError();
 }
 
-int launch_verify(int mode, const VerifyArgs &a, hipStream_t st) {
+int launch_verify(int mode, const VerifyArgs &a, hipStream_t st, hipEvent_t *ev) {
     switch (mode) {
-        case 2: return verify_impl<Params<2>>(a, st);
-        case 3: return verify_impl<Params<3>>(a, st);
-        case 5: return verify_impl<Params<5>>(a, st);
+        case 2: return verify_impl<Params<2>>(a, st, ev);
+        case 3: return verify_impl<Params<3>>(a, st, ev);
+        case 5: return verify_impl<Params<5>>(a, st, ev);
         default: return -1;
     }
 }

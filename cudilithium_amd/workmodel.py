@@ -67,6 +67,18 @@ def verify_ops(mode: int, mlen: int) -> int:
     return _ops(perms, l + 1 + k, k, k * l + k)
 
 
+def verify_kernel_ops(mode: int, mlen: int) -> dict:
+    """verify_ops split by the engine's three launches: vf_prep (tr, mu,
+    SampleInBall), vf_main (ExpandA, NTT of z / c / t1, A z - c t1, INTT) and
+    vf_final (c~' = H(mu || w1'))."""
+    k, l, w1b, pkb = PARAMS[mode]
+    prep = _ops(_hash_perms(pkb) + _hash_perms(32 + mlen) + 1, 0, 0, 0)
+    main = _ops(5 * k * l, l + 1 + k, k, k * l + k)
+    final = _ops(_hash_perms(64 + k * w1b), 0, 0, 0)
+    assert prep + main + final == verify_ops(mode, mlen)
+    return {"vf_prep": prep, "vf_main": main, "vf_final": final}
+
+
 def keygen_ops(mode: int) -> int:
     k, l, _, pkb = PARAMS[mode]
     perms = 1 + 1 + 2 * (l + k) + 5 * k * l + _hash_perms(pkb)

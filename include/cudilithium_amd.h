@@ -99,9 +99,12 @@ size_t dil_reference_pool_row_bytes(int mode, int which, size_t mlen);
 /* Host buffers of the three drop-in calls may be pinned (hipHostMalloc) or
  * pageable.  Pinned rows are copied by a kernel on `stream` through their
  * device mapping, asynchronously; pageable rows go through a 32 MiB pinned
- * staging buffer that the library allocates on first use and keeps until the
- * process ends (packed by up to 4 host threads), and those copies complete
- * before the call returns. */
+ * staging buffer that the library allocates on first use and keeps until
+ * dil_release_host_staging or the end of the process (packed by up to 4
+ * host threads), and those copies complete before the call returns. */
+/* Frees the pinned staging buffer of pageable copies (after any copy that
+ * holds it); the next pageable copy allocates it again.  0 or hipError_t. */
+int dil_release_host_staging(void);
 
 /* crypto_sign_keypair (api.cuh:27-29, api.cu:26-61): batch keys whose seeds
  * are SHAKE128(le64(rand_index + 2i + 1))[0:32]; pk/sk are host buffers
@@ -134,6 +137,26 @@ int dil_crypto_sign_signature_perstage(int mode, uint8_t *sig, size_t sig_pitch,
                                        size_t d_sign_mem_pool_pitch, uint8_t *d_temp_mem_pool,
                                        size_t d_temp_mem_pool_pitch, dil_task_lut *lut,
                                        size_t exec_threshold, size_t batch_size, void *stream);
+
+/* crypto_sign_signature as built with -DDILITHIUM_RANDOMIZED_SIGNING
+ * (api.cu:143-148): per item, rhoprime is 64 bytes drawn from the random
+ * source (one draw of 64 * batch_size bytes, item i taking bytes
+ * [64 i, 64 i + 64)) instead of SHAKE256(key || mu); otherwise the same
+ * contract as dil_crypto_sign_signature (persistent engine).  Returns the
+ * random source's error, if it reports one, before any GPU work. */
+int dil_crypto_sign_signature_randomized(int mode, uint8_t *sig, size_t sig_pitch, size_t *siglen,
+                                         const uint8_t *m, size_t m_pitch, size_t mlen,
+                                         const uint8_t *sk, uint8_t *d_sign_mem_pool,
+                                         size_t d_sign_mem_pool_pitch, uint8_t *d_temp_mem_pool,
+                                         size_t d_temp_mem_pool_pitch, dil_task_lut *lut,
+                                         size_t exec_threshold, size_t batch_size, void *stream);
+/* The random source of randomized signing (the reference's randombytes,
+ * benches/trng.c:21-80, which reads the OS generator).  The default reads
+ * getrandom(2); dil_set_randombytes installs the caller's function (NULL
+ * restores the default).  A source returns 0 on success.  Process-wide. */
+typedef int (*dil_randombytes_fn)(uint8_t *out, size_t outlen);
+void dil_set_randombytes(dil_randombytes_fn fn);
+int dil_randombytes(uint8_t *out, size_t outlen);
 
 /* crypto_sign_verify (api.cuh:58-63, api.cu:454-515): ret[i] = 0 for a valid
  * signature, -1 otherwise; siglen != CRYPTO_BYTES sets every ret to -1
@@ -195,6 +218,13 @@ int dil_verify_device(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig
                       const uint8_t *d_pk, size_t pk_pitch, const uint8_t *d_msg,
                       const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, uint8_t *d_work,
                       size_t work_pitch, void *stream);
+/* dil_verify_device with its three launches (vf_prep, vf_main, vf_final)
+ * timed by HIP events on `stream`; waits for the call and writes the three
+ * durations in ms to kernel_ms3.  A measurement aid, same results. */
+int dil_verify_device_timed(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch,
+                            const uint8_t *d_pk, size_t pk_pitch, const uint8_t *d_msg,
+                            const uint64_t *d_moff, const uint32_t *d_mlen, size_t n,
+                            uint8_t *d_work, size_t work_pitch, float *kernel_ms3, void *stream);
 
 /* Signing context: owns the per-item rows (dil_sign_pool_row_bytes(mode, 0)
  * each), nslots attempt slots and the scheduler tables of either engine. */

@@ -8,6 +8,12 @@
 // return the library's status codes (0 on success) where the reference always
 // returns 0; crypto_sign_verify returns that status too (void in the
 // reference; ignoring it is source compatible).
+//
+// -DDILITHIUM_RANDOMIZED_SIGNING selects randomized signing as in the
+// reference (api.cu:143-148): rhoprime comes from the library's random source
+// (getrandom(2) unless the caller installs its own function with
+// dil_set_randombytes, the counterpart of linking the reference's randombytes,
+// benches/trng.c:21-80).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -124,9 +130,15 @@ inline int crypto_sign_signature(uint8_t *sig, size_t sig_pitch, size_t *siglen,
     dil_task_lut c{reinterpret_cast<dil_sign_lut_element *>(lut.sign_lut.data()), lut.sign_lut.size(),
                    lut.h_exec_lut, lut.h_done_lut, reinterpret_cast<dil_copy_lut_element *>(lut.h_copy_lut),
                    lut.d_exec_lut, lut.d_done_lut, reinterpret_cast<dil_copy_lut_element *>(lut.d_copy_lut)};
+#ifdef DILITHIUM_RANDOMIZED_SIGNING
+    return dil_crypto_sign_signature_randomized(DILITHIUM_MODE, sig, sig_pitch, siglen, m, m_pitch, mlen, sk,
+                                                d_sign_mem_pool, d_sign_mem_pool_pitch, d_temp_mem_pool,
+                                                d_temp_mem_pool_pitch, &c, exec_threshold, batch_size, stream);
+#else
     return dil_crypto_sign_signature(DILITHIUM_MODE, sig, sig_pitch, siglen, m, m_pitch, mlen, sk, d_sign_mem_pool,
                                      d_sign_mem_pool_pitch, d_temp_mem_pool, d_temp_mem_pool_pitch, &c,
                                      exec_threshold, batch_size, stream);
+#endif
 }
 
 inline int crypto_sign_verify(int *ret, const uint8_t *sig, size_t sig_pitch, size_t siglen, const uint8_t *m,

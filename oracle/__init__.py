@@ -14,6 +14,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# compiler flags of liboracle.so (oracle/Makefile): a portable x86-64-v3
+# build, since the library is compiled on one host and loaded on another
+BUILD_FLAGS = "gcc -O3 -march=x86-64-v3 -fopenmp"
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _i32p = ctypes.POINTER(ctypes.c_int32)

@@ -107,3 +107,11 @@ def test_dropin_pageable_copies_span_staging_chunks():
                               lut, thr, n)
     api.stream_sync(None)
     assert np.array_equal(hsig.array, sig0)
+    # the staging buffer can be released between calls; the next pageable copy
+    # allocates it again and gives the same bytes
+    assert api.L().dil_release_host_staging() == 0
+    assert api.L().dil_release_host_staging() == 0
+    sig1 = np.zeros((n, sigb), np.uint8)
+    api.crypto_sign_signature(mode, sig1, sigb, m, mlen, mlen, sk0, sp, sp.pitch, tp, tp.pitch, lut, thr, n)
+    api.stream_sync(None)
+    assert np.array_equal(sig1, sig0)
