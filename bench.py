@@ -443,9 +443,12 @@ def bench_reference_conditions(args, n_streams: int = 10):
     pool = ThreadPoolExecutor(max_workers=n_streams)
 
     def keypair_s():
+        # rand_index 2 f: the slice's keys equal the whole-batch call's (the
+        # reference passes 0 on every stream, repeating the first slice's keys)
         for i, (f, c) in enumerate(slices):
-            api.L().dil_crypto_sign_keypair(mode, hpk.ptr + f * pkb, hsk.ptr + f * skb, kpool.ptr + f * kpool.pitch,
-                                            kpool.pitch, c, streams[i].handle, 0)
+            api._lib.check(api.L().dil_crypto_sign_keypair(mode, hpk.ptr + f * pkb, hsk.ptr + f * skb,
+                                                           kpool.ptr + f * kpool.pitch, kpool.pitch, c,
+                                                           streams[i].handle, 2 * f), "keypair")
 
     def sign_one(i):
         f, c = slices[i]

@@ -22,6 +22,8 @@ POOL_KEYPAIR, POOL_SIGN, POOL_SIGN_TEMP, POOL_VERIFY = 0, 1, 2, 3
 ENGINE_PERSISTENT, ENGINE_PERSTAGE = 0, 1
 # dil_sign_ctx_status bits
 STATUS_ROUND_CAP, STATUS_NO_SECRET = 1, 2
+# message hashing flag of dil_sign_ctx_set_message_hashing / dil_verify_device*_ex
+HASH_LONG_MESSAGES = 1
 
 
 class SignLutElement(ctypes.Structure):  # api.cuh:31-34
@@ -85,6 +87,13 @@ def _register():
     R("dil_keystore_mode", c_int, [c_void_p])
     R("dil_keystore_has_secret", c_int, [c_void_p, c_size_t])
     R("dil_release_host_staging", c_int, [])
+    R("dil_set_kernel_tiling", c_int, [c_int, c_int, c_int, c_int])
+    R("dil_get_kernel_tiling", c_int, [c_int, c_int, c_size_t])
+    R("dil_sign_ctx_set_message_hashing", c_int, [c_void_p, c_int])
+    R("dil_verify_device_ex", c_int, [c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p,
+                                      c_void_p, c_size_t, c_void_p, c_size_t, c_int, c_void_p])
+    R("dil_verify_device_resident_ex", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p,
+                                               c_void_p, c_size_t, c_void_p, c_size_t, c_int, c_void_p])
     R("dil_keystore_load", c_int, [c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t, c_void_p])
     R("dil_keystore_keygen", c_int, [c_void_p, c_size_t, c_void_p, c_size_t, ctypes.c_uint64, c_size_t, c_void_p])
     R("dil_keystore_export", c_int, [c_void_p, c_size_t, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p])
@@ -105,9 +114,20 @@ def _register():
     _registered = True
 
 
+_tuned = False
+
+
 def L():
+    """The library, with its C signatures registered and the autotuner's
+    recorded kernel tilings (tuning.json) applied on first use."""
+    global _tuned
     _register()
-    return _lib.load()
+    lib = _lib.load()
+    if not _tuned:
+        _tuned = True
+        from . import autotune
+        autotune.apply_cache(lib)
+    return lib
 
 
 def sizes(mode: int):
@@ -337,6 +357,10 @@ class SignContext:
     def set_tiling(self, slots_per_block: int = 0, occupancy: int = 0, blocks: int = 0) -> None:
         _lib.check(L().dil_sign_ctx_set_tiling(self.handle, slots_per_block, occupancy, blocks), "set_tiling")
         self.slots_per_block, self.occupancy = slots_per_block, occupancy
+
+    def set_message_hashing(self, flags: int) -> None:
+        """0, or HASH_LONG_MESSAGES: mu / rhoprime with one sponge per 16-lane row."""
+        _lib.check(L().dil_sign_ctx_set_message_hashing(self.handle, int(flags)), "set_message_hashing")
 
     def set_claim(self, items_per_block: int) -> None:
         _lib.check(L().dil_sign_ctx_set_claim(self.handle, int(items_per_block)), "set_claim")

@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <functional>
 #include <mutex>
 #include <utility>
@@ -55,6 +56,29 @@ AuxStreams *aux_streams(hipStream_t caller) {
     return a;
 }
 
+namespace {
+// [kind][mode slot 0..2 for modes 2/3/5][bucket]
+std::atomic<int> g_tiling[TILE_KINDS][3][TILE_BUCKETS];
+
+int mode_slot(int mode) { return mode == 2 ? 0 : mode == 3 ? 1 : mode == 5 ? 2 : -1; }
+}  // namespace
+
+int tiling_bucket(size_t n) { return n <= 1024 ? 0 : n <= 4096 ? 1 : n <= 16384 ? 2 : 3; }
+
+int tiling_get(int kind, int mode, size_t n) {
+    const int ms = kind == TILE_NTT_WAVES ? 0 : mode_slot(mode);
+    if (kind < 0 || kind >= TILE_KINDS || ms < 0) return 0;
+    return g_tiling[kind][ms][tiling_bucket(n)].load(std::memory_order_relaxed);
+}
+
+int tiling_set(int kind, int mode, int bucket, int value) {
+    const int ms = kind == TILE_NTT_WAVES ? 0 : mode_slot(mode);
+    if (kind < 0 || kind >= TILE_KINDS || ms < 0 || bucket < -1 || bucket >= TILE_BUCKETS || value < 0) return -1;
+    for (int b = 0; b < TILE_BUCKETS; ++b)
+        if (bucket < 0 || b == bucket) g_tiling[kind][ms][b].store(value, std::memory_order_relaxed);
+    return 0;
+}
+
 int default_grid_blocks(size_t units, int per_block, int blocks_per_cu) {
     size_t need = (units + per_block - 1) / per_block;
     size_t cap = (size_t)device_cu_count() * blocks_per_cu;
@@ -70,6 +94,20 @@ using namespace dil;
 static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 extern "C" {
+
+int dil_set_kernel_tiling(int kind, int mode, int bucket, int value) {
+    bool ok = false;
+    switch (kind) {
+        case DIL_TILE_KEYGEN_ITEMS: ok = keygen_items_supported(mode, value); break;
+        case DIL_TILE_VERIFY_ITEMS: ok = verify_items_supported(mode, value); break;
+        case DIL_TILE_NTT_WAVES: ok = ntt_waves_supported(value); break;
+        default: ok = false;
+    }
+    if (!ok) return DIL_EINVAL;
+    return tiling_set(kind, mode, bucket, value) ? DIL_EINVAL : 0;
+}
+
+int dil_get_kernel_tiling(int kind, int mode, size_t n) { return tiling_get(kind, mode, n); }
 
 size_t dil_public_key_bytes(int mode) {
     switch (mode) {

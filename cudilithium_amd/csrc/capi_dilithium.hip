@@ -575,16 +575,25 @@ int dil_verify_device_timed(int mode, int32_t *d_ret, const uint8_t *d_sig, size
     return e;
 }
 
-int dil_verify_device(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_pk,
-                      size_t pk_pitch, const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen,
-                      size_t n, uint8_t *d_work, size_t work_pitch, void *stream) {
+int dil_verify_device_ex(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_pk,
+                         size_t pk_pitch, const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen,
+                         size_t n, uint8_t *d_work, size_t work_pitch, int flags, void *stream) {
     if (!valid_mode(mode)) return DIL_EINVAL;
     if (n == 0) return 0;
+    if (flags & ~DIL_HASH_LONG_MESSAGES) return DIL_EINVAL;
     if (!d_ret || !d_sig || !d_pk || !d_work || !d_moff || !d_mlen || !d_msg) return DIL_EINVAL;
     if (sig_pitch < dil_signature_bytes(mode) || pk_pitch < dil_public_key_bytes(mode)) return DIL_EINVAL;
     if (work_pitch < dil_verify_work_row_bytes(mode) || (work_pitch & 255)) return DIL_ENOMEM;
     VerifyArgs a{d_sig, sig_pitch, d_pk, pk_pitch, MsgRef{d_msg, d_moff, 0, d_mlen, 0}, d_work, work_pitch, d_ret, 1, n};
+    a.row_mu = (flags & DIL_HASH_LONG_MESSAGES) ? 1 : 0;
     return launch_verify(mode, a, S(stream));
+}
+
+int dil_verify_device(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_pk,
+                      size_t pk_pitch, const uint8_t *d_msg, const uint64_t *d_moff, const uint32_t *d_mlen,
+                      size_t n, uint8_t *d_work, size_t work_pitch, void *stream) {
+    return dil_verify_device_ex(mode, d_ret, d_sig, sig_pitch, d_pk, pk_pitch, d_msg, d_moff, d_mlen, n, d_work,
+                                work_pitch, 0, stream);
 }
 
 }  // extern "C"
@@ -611,6 +620,7 @@ struct dil_sign_ctx {
     bool resident_only = false;  // rows without the per-item expanded arrays
     int claim = 0;               // items owned per workgroup (0 = slots per workgroup)
     bool profile = false;        // phase sums into d_head words 16..27
+    bool row_mu = false;         // long messages: mu / rhoprime with one sponge per 16-lane row
 };
 
 // Resident keys: KeyLayout rows (pk, sk, tr, A, s1^, s2^, t0^, t1^).
@@ -656,6 +666,7 @@ static int sign_enqueue(dil_sign_ctx *c, uint8_t *d_sig, size_t sig_pitch, const
         a.n = n;
         a.rnd = d_rnd;
         a.rnd_pitch = rnd_pitch;
+        a.row_mu = c->row_mu ? 1 : 0;
         if (ks) {
             a.key_idx = d_key_idx;
             a.key_ints = ks->rows + KeyLayout<P>::MAT;
@@ -816,6 +827,12 @@ int dil_sign_ctx_executed_attempts(dil_sign_ctx *c, uint64_t *count, void *strea
     if (!e) e = (int)hipStreamSynchronize(S(stream));
     *count = v;
     return e;
+}
+
+int dil_sign_ctx_set_message_hashing(dil_sign_ctx *c, int flags) {
+    if (!c || (flags & ~DIL_HASH_LONG_MESSAGES)) return DIL_EINVAL;
+    c->row_mu = (flags & DIL_HASH_LONG_MESSAGES) != 0;
+    return 0;
 }
 
 int dil_sign_ctx_profile_phases(dil_sign_ctx *c, int enable) {
@@ -1057,19 +1074,29 @@ int dil_sign_device_resident(dil_sign_ctx *c, const dil_keystore *k, const uint3
                         d_key_idx);
 }
 
-int dil_verify_device_resident(const dil_keystore *k, const uint32_t *d_key_idx, int32_t *d_ret,
-                               const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_msg,
-                               const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, uint8_t *d_work,
-                               size_t work_pitch, void *stream) {
+int dil_verify_device_resident_ex(const dil_keystore *k, const uint32_t *d_key_idx, int32_t *d_ret,
+                                  const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_msg,
+                                  const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, uint8_t *d_work,
+                                  size_t work_pitch, int flags, void *stream) {
     if (!k || !d_key_idx) return DIL_EINVAL;
     if (n == 0) return 0;
+    if (flags & ~DIL_HASH_LONG_MESSAGES) return DIL_EINVAL;
     const int mode = k->mode;
     if (!d_ret || !d_sig || !d_work || !d_moff || !d_mlen || !d_msg) return DIL_EINVAL;
     if (sig_pitch < dil_signature_bytes(mode)) return DIL_EINVAL;
     if (work_pitch < dil_verify_work_row_bytes(mode) || (work_pitch & 255)) return DIL_ENOMEM;
     VerifyArgs a{d_sig, sig_pitch, k->rows, k->pitch, MsgRef{d_msg, d_moff, 0, d_mlen, 0}, d_work, work_pitch,
                  d_ret, 1, n, d_key_idx, k->rows, k->pitch, (uint32_t)k->capacity};
+    a.row_mu = (flags & DIL_HASH_LONG_MESSAGES) ? 1 : 0;
     return launch_verify(mode, a, S(stream));
+}
+
+int dil_verify_device_resident(const dil_keystore *k, const uint32_t *d_key_idx, int32_t *d_ret,
+                               const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_msg,
+                               const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, uint8_t *d_work,
+                               size_t work_pitch, void *stream) {
+    return dil_verify_device_resident_ex(k, d_key_idx, d_ret, d_sig, sig_pitch, d_msg, d_moff, d_mlen, n, d_work,
+                                         work_pitch, 0, stream);
 }
 
 }  // extern "C"

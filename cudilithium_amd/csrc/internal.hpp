@@ -34,6 +34,19 @@ struct AuxStreams {
 };
 AuxStreams *aux_streams(hipStream_t caller);
 
+// Kernel tilings chosen at run time (set by the autotuner's cache through
+// dil_set_kernel_tiling): per (kind, mode, batch bucket) a value, 0 for the
+// compiled default.  Buckets are batch sizes up to 1K, 4K, 16K and above.
+enum TilingKind { TILE_KEYGEN_ITEMS = 0, TILE_VERIFY_ITEMS = 1, TILE_NTT_WAVES = 2, TILE_KINDS = 3 };
+constexpr int TILE_BUCKETS = 4;
+int tiling_bucket(size_t n);
+int tiling_get(int kind, int mode, size_t n);  // 0: the compiled default
+int tiling_set(int kind, int mode, int bucket, int value);
+// whether a value (0 = default) is one the kernels are built for
+bool keygen_items_supported(int mode, int g);
+bool verify_items_supported(int mode, int g);
+bool ntt_waves_supported(int w);
+
 // Launchers (return a hipError_t value, 0 on success).
 int launch_ntt(int kind, int32_t *c, const int32_t *a, const int32_t *b, size_t npoly, size_t pitch_words,
                int blocks, hipStream_t stream);

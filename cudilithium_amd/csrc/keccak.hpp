@@ -222,6 +222,114 @@ __device__ __forceinline__ void shake_absorb_regs(Lane64 (&s)[25], const uint64_
     s[RW - 1].hi ^= 0x80000000u;
 }
 
+// ---------------------------------------------------------------- lane pairs
+// Keccak-f[1600] over a pair of lanes (2j, 2j + 1): lane 2j + h holds half h
+// (h = 0: bits 0-31, h = 1: bits 32-63) of all 25 state words in 25 VGPRs.
+// Bitwise steps (theta parities and update, chi, iota) need only the own
+// half; a 64-bit rotation needs the other half, read from the partner lane
+// with one DPP move (quad_perm [1,0,3,2]), and is then one funnel shift, the
+// same instruction for both halves.  A round is about 125 VALU instructions
+// against 180 for one state per lane, and a wave holds 32 states instead of
+// 64: for a handful of serial sponges (the challenge hash of a few dozen
+// signing attempts) the chain is a third shorter, with the same lane count.
+
+// value of the partner lane (lane ^ 1)
+__device__ __forceinline__ uint32_t pair_other(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);  // quad_perm(1, 0, 3, 2)
+}
+
+// own half of rotl64(word, R) for a word whose own / other halves are given
+template <int R>
+__device__ __forceinline__ uint32_t rotl_half(uint32_t own, uint32_t other) {
+    if constexpr (R == 0) {
+        return own;
+    } else if constexpr (R == 32) {
+        return other;
+    } else if constexpr (R < 32) {
+        return funnel(own, other, 32 - R);
+    } else {
+        return funnel(other, own, 64 - R);
+    }
+}
+
+template <int I>
+__device__ __forceinline__ void rho_pi_pair_step(const uint32_t (&a)[25], uint32_t (&b)[25]) {
+    if constexpr (kKeccak.rho[I] == 0)
+        b[kKeccak.pi[I]] = a[I];
+    else
+        b[kKeccak.pi[I]] = rotl_half<kKeccak.rho[I]>(a[I], pair_other(a[I]));
+    if constexpr (I + 1 < 25) rho_pi_pair_step<I + 1>(a, b);
+}
+
+__device__ __forceinline__ void keccak_round_pair(uint32_t (&a)[25], uint32_t rc_half) {
+    uint32_t c[5], co[5], b[25];
+#pragma unroll
+    for (int x = 0; x < 5; ++x)
+        c[x] = bitop3<kXor3>(bitop3<kXor3>(a[x], a[x + 5], a[x + 10]), a[x + 15], a[x + 20]);
+#pragma unroll
+    for (int x = 0; x < 5; ++x) co[x] = pair_other(c[x]);
+#pragma unroll
+    for (int x = 0; x < 5; ++x) {
+        const uint32_t r = rotl_half<1>(c[(x + 1) % 5], co[(x + 1) % 5]);
+#pragma unroll
+        for (int y = 0; y < 5; ++y) a[x + 5 * y] = bitop3<kXor3>(a[x + 5 * y], c[(x + 4) % 5], r);
+    }
+    rho_pi_pair_step<0>(a, b);
+#pragma unroll
+    for (int y = 0; y < 5; ++y)
+#pragma unroll
+        for (int x = 0; x < 5; ++x)
+            a[x + 5 * y] = bitop3<kChi>(b[x + 5 * y], b[(x + 1) % 5 + 5 * y], b[(x + 2) % 5 + 5 * y]);
+    a[0] ^= rc_half;
+}
+
+// h: this lane's half (lane & 1); both lanes of the pair must be active
+__device__ __forceinline__ void keccak_f1600_pair(uint32_t (&a)[25], uint32_t h) {
+#pragma unroll 2
+    for (int r = 0; r < 24; ++r) {
+        const uint64_t rc = kKeccakRC[r];
+        keccak_round_pair(a, h ? (uint32_t)(rc >> 32) : (uint32_t)rc);
+    }
+}
+
+// SHAKE absorb of LEN bytes (a multiple of 8, known at compile time) in the
+// pair layout: get(w) returns this lane's half of input word w.  Leaves the
+// state padded for the first squeeze (the pair counterpart of shake_absorb).
+template <int RATE, int LEN, class Get>
+__device__ __forceinline__ void shake_absorb_pair(uint32_t (&s)[25], Get get, uint32_t h) {
+    constexpr int RW = RATE / 8, NWORDS = LEN / 8, FULL = NWORDS / RW, REST = NWORDS % RW;
+    static_assert(LEN % 8 == 0, "whole words");
+#pragma unroll
+    for (int i = 0; i < 25; ++i) s[i] = 0u;
+    uint32_t cur[RW];
+    if constexpr (FULL > 0) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i) cur[i] = get(i);
+    }
+#pragma unroll 1
+    for (int blk = 0; blk < FULL; ++blk) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i) s[i] ^= cur[i];
+        if (blk + 1 < FULL) {
+#pragma unroll
+            for (int i = 0; i < RW; ++i) cur[i] = get((uint32_t)((blk + 1) * RW + i));
+        }
+        keccak_f1600_pair(s, h);
+    }
+#pragma unroll
+    for (int i = 0; i < REST; ++i) s[i] ^= get((uint32_t)(FULL * RW + i));
+    // domain byte 0x1F at byte 0 of word REST (low half), final bit 63 of word RW - 1
+    if (h == 0) s[REST] ^= 0x1Fu;
+    if (h == 1) s[RW - 1] ^= 0x80000000u;
+}
+
+// full 64-bit word i of the state (both lanes of the pair get it)
+__device__ __forceinline__ uint64_t pair_word(const uint32_t (&s)[25], int i, uint32_t h) {
+    const uint32_t own = s[i], other = pair_other(own);
+    const uint32_t lo = h ? other : own, hi = h ? own : other;
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Byte i of the current output block (i compile-time after unrolling).
 __device__ __forceinline__ uint32_t state_byte(const Lane64 (&s)[25], int i) {
     const Lane64 w = s[i >> 3];

@@ -89,27 +89,27 @@ __global__ __launch_bounds__(256) void kg_eta_kernel(KeygenArgs a) {
     }
 }
 
+// keys per workgroup of kg_main: the compiled default, and the values a
+// tiling may choose (those whose LDS fits in 160 KiB)
 template <class P>
 struct KgMainCfg {
     static constexpr int G = P::MODE == 2 ? 16 : (P::MODE == 3 ? 8 : 4);
 };
 
-template <class P>
+template <class P, int G>
 struct KgShared {
-    static constexpr int G = KgMainCfg<P>::G;
     ZetaLds zt;
     int32_t x[4][NTT_XBUF];
     int32_t shat[G][P::L][N];
     int32_t acc[G][P::K][N];
 };
 
-template <class P>
+template <class P, int G>
 __global__ __launch_bounds__(256) void kg_main_kernel(KeygenArgs a) {
     using Z = Sizes<P>;
     using KL = KeypairLayout<P>;
-    constexpr int G = KgMainCfg<P>::G;
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    KgShared<P> &sh = *reinterpret_cast<KgShared<P> *>(smem_raw);
+    KgShared<P, G> &sh = *reinterpret_cast<KgShared<P, G> *>(smem_raw);
     const size_t item0 = (size_t)blockIdx.x * G;
     const int wave = threadIdx.x >> 6;
     zeta_lds_load(sh.zt);
@@ -190,22 +190,65 @@ __global__ __launch_bounds__(256) void kg_tr_kernel(KeygenArgs a) {
     for (int w = 0; w < 4; ++w) store_le64(sk + 8 * w, tr[w]);
 }
 
+template <class P, int G>
+constexpr bool kg_fits() {
+    return sizeof(KgShared<P, G>) <= 160 * 1024;
+}
+
+template <class P, int G>
+static int launch_kg_main(const KeygenArgs &a, hipStream_t st) {
+    if constexpr (!kg_fits<P, G>()) {
+        return -1;
+    } else {
+        const size_t smem = sizeof(KgShared<P, G>);
+        // set once per kernel: a host call, not a stream operation
+        static const hipError_t attr = hipFuncSetAttribute((const void *)kg_main_kernel<P, G>,
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (attr != hipSuccess) return (int)attr;
+        hipLaunchKernelGGL((kg_main_kernel<P, G>), dim3((unsigned)((a.n + G - 1) / G)), dim3(256), smem, st, a);
+        return 0;
+    }
+}
+
 template <class P>
 static int keygen_impl(const KeygenArgs &a, hipStream_t st) {
     if (a.n == 0) return 0;
     constexpr int NP = P::L + P::K;
-    constexpr int G = KgMainCfg<P>::G;
     const unsigned b1 = (unsigned)((a.n + 255) / 256);
     hipLaunchKernelGGL(kg_seed_kernel<P>, dim3(b1), dim3(256), 0, st, a);
     hipLaunchKernelGGL(kg_eta_kernel<P>, dim3((unsigned)((a.n * NP + 255) / 256)), dim3(256), 0, st, a);
-    const size_t smem = sizeof(KgShared<P>);
-    // set once per kernel: a host call, not a stream operation
-    static const hipError_t attr = hipFuncSetAttribute((const void *)kg_main_kernel<P>,
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (attr != hipSuccess) return (int)attr;
-    hipLaunchKernelGGL(kg_main_kernel<P>, dim3((unsigned)((a.n + G - 1) / G)), dim3(256), smem, st, a);
+    int e;
+    switch (tiling_get(TILE_KEYGEN_ITEMS, P::MODE, a.n)) {
+        case 2: e = launch_kg_main<P, 2>(a, st); break;
+        case 4: e = launch_kg_main<P, 4>(a, st); break;
+        case 8: e = launch_kg_main<P, 8>(a, st); break;
+        case 16: e = launch_kg_main<P, 16>(a, st); break;
+        default: e = launch_kg_main<P, KgMainCfg<P>::G>(a, st); break;
+    }
+    if (e) return e;
     hipLaunchKernelGGL(kg_tr_kernel<P>, dim3(b1), dim3(256), 0, st, a);
     return (int)hipGetLastError();
+}
+
+template <class P>
+static bool keygen_items_ok(int g) {
+    switch (g) {
+        case 0: return true;
+        case 2: return kg_fits<P, 2>();
+        case 4: return kg_fits<P, 4>();
+        case 8: return kg_fits<P, 8>();
+        case 16: return kg_fits<P, 16>();
+        default: return false;
+    }
+}
+
+bool keygen_items_supported(int mode, int g) {
+    switch (mode) {
+        case 2: return keygen_items_ok<Params<2>>(g);
+        case 3: return keygen_items_ok<Params<3>>(g);
+        case 5: return keygen_items_ok<Params<5>>(g);
+        default: return false;
+    }
 }
 
 int launch_keygen(int mode, const KeygenArgs &a, hipStream_t st) {

@@ -13,11 +13,13 @@
 
 namespace dil {
 
-constexpr int NTT_WAVES_PER_BLOCK = 4;
+// wavefronts per workgroup: 4 by default, or 2 / 8 as a tiling chooses
+constexpr int NTT_WAVES_DEFAULT = 4;
 
+template <int W>
 struct NttShared {
     ZetaLds zt;
-    int32_t x[NTT_WAVES_PER_BLOCK][NTT_XBUF];
+    int32_t x[W][NTT_XBUF];
 };
 
 __device__ __forceinline__ void load_A(int32_t (&v)[4], const int32_t *p, int l) {
@@ -41,18 +43,17 @@ __device__ __forceinline__ void store_D(const int32_t (&v)[4], int32_t *p, int l
 
 // mode 0: forward in place; 1: inverse in place;
 // 2: c = INTT(NTT(a) o_mont b) with b already in the NTT domain.
-template <int KIND>
-__global__ __launch_bounds__(64 * NTT_WAVES_PER_BLOCK) void ntt_batch_kernel(int32_t *c, const int32_t *a,
-                                                                            const int32_t *b, size_t npoly,
-                                                                            size_t pitch) {
-    __shared__ NttShared sh;
+template <int KIND, int W>
+__global__ __launch_bounds__(64 * W) void ntt_batch_kernel(int32_t *c, const int32_t *a, const int32_t *b,
+                                                           size_t npoly, size_t pitch) {
+    __shared__ NttShared<W> sh;
     zeta_lds_load(sh.zt);
     __syncthreads();
     const int l = lane_id();
     const int w = threadIdx.x >> 6;
     int32_t *x = sh.x[w];
-    const size_t nwaves = (size_t)gridDim.x * NTT_WAVES_PER_BLOCK;
-    for (size_t p = (size_t)blockIdx.x * NTT_WAVES_PER_BLOCK + w; p < npoly; p += nwaves) {
+    const size_t nwaves = (size_t)gridDim.x * W;
+    for (size_t p = (size_t)blockIdx.x * W + w; p < npoly; p += nwaves) {
         int32_t v[4];
         if constexpr (KIND == 0) {
             load_A(v, a + p * pitch, l);
@@ -75,23 +76,35 @@ __global__ __launch_bounds__(64 * NTT_WAVES_PER_BLOCK) void ntt_batch_kernel(int
     }
 }
 
-int launch_ntt(int kind, int32_t *c, const int32_t *a, const int32_t *b, size_t npoly, size_t pitch_words,
-               int blocks, hipStream_t stream) {
-    if (npoly == 0) return 0;
-    if (blocks <= 0) blocks = default_grid_blocks(npoly, NTT_WAVES_PER_BLOCK);
-    dim3 grid(blocks), block(64 * NTT_WAVES_PER_BLOCK);
+template <int W>
+static int launch_ntt_w(int kind, int32_t *c, const int32_t *a, const int32_t *b, size_t npoly, size_t pitch_words,
+                        int blocks, hipStream_t stream) {
+    if (blocks <= 0) blocks = default_grid_blocks(npoly, W, 32 / W);
+    dim3 grid(blocks), block(64 * W);
     switch (kind) {
         case 0:
-            hipLaunchKernelGGL(ntt_batch_kernel<0>, grid, block, 0, stream, c, a, b, npoly, pitch_words);
+            hipLaunchKernelGGL((ntt_batch_kernel<0, W>), grid, block, 0, stream, c, a, b, npoly, pitch_words);
             break;
         case 1:
-            hipLaunchKernelGGL(ntt_batch_kernel<1>, grid, block, 0, stream, c, a, b, npoly, pitch_words);
+            hipLaunchKernelGGL((ntt_batch_kernel<1, W>), grid, block, 0, stream, c, a, b, npoly, pitch_words);
             break;
         default:
-            hipLaunchKernelGGL(ntt_batch_kernel<2>, grid, block, 0, stream, c, a, b, npoly, pitch_words);
+            hipLaunchKernelGGL((ntt_batch_kernel<2, W>), grid, block, 0, stream, c, a, b, npoly, pitch_words);
             break;
     }
     return (int)hipGetLastError();
+}
+
+bool ntt_waves_supported(int w) { return w == 0 || w == 2 || w == 4 || w == 8; }
+
+int launch_ntt(int kind, int32_t *c, const int32_t *a, const int32_t *b, size_t npoly, size_t pitch_words,
+               int blocks, hipStream_t stream) {
+    if (npoly == 0) return 0;
+    switch (tiling_get(TILE_NTT_WAVES, 0, npoly)) {
+        case 2: return launch_ntt_w<2>(kind, c, a, b, npoly, pitch_words, blocks, stream);
+        case 8: return launch_ntt_w<8>(kind, c, a, b, npoly, pitch_words, blocks, stream);
+        default: return launch_ntt_w<NTT_WAVES_DEFAULT>(kind, c, a, b, npoly, pitch_words, blocks, stream);
+    }
 }
 
 // Keccak-f[1600] over n independent states (one per lane), for testing the

@@ -54,6 +54,9 @@ struct VerifyArgs {
     const uint8_t *keys;
     size_t key_pitch;
     uint32_t nkeys;
+    // Long messages: tr and mu are computed before vf_prep with one sponge
+    // per 16-lane row (vf_mu_row_kernel) and vf_prep skips them.
+    int row_mu;
 };
 
 // Host-side per-item state of the per-stage scheduler, laid out as the
@@ -73,6 +76,10 @@ int launch_key_expand(int mode, uint8_t *keys, size_t key_pitch, size_t n, int w
 // ev: null, or four events recorded around vf_prep / vf_main / vf_final
 int launch_verify(int mode, const VerifyArgs &a, hipStream_t st, hipEvent_t *ev = nullptr);
 int launch_sign_fixed(int mode, const SignArgs &a, hipStream_t st);
+// mu / rhoprime (signing) and tr / mu (verification) with one sponge per
+// 16-lane row instead of one lane: for batches of long messages
+int launch_sign_mu_row(int mode, const SignArgs &a, hipStream_t st);
+int launch_verify_mu_row(int mode, const VerifyArgs &a, hipStream_t st);
 int launch_sign_round(int mode, const SignArgs &a, const SlotRef &sr, const uint32_t *d_exec, size_t nslot,
                       int by_item, uint8_t *d_done, hipStream_t st);
 int launch_sign_copy(int mode, const SignArgs &a, const SlotRef &sr, const uint64_t *d_copy, size_t ncopy,

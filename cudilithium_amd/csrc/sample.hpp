@@ -136,8 +136,10 @@ __device__ __forceinline__ uint32_t mask_word(const uint32_t (&a)[8], uint32_t w
     return b2 ? u1 : u0;
 }
 
+// store: false computes the masks without writing them (a lane that runs
+// the same challenge as its pair partner, which writes)
 template <int TAU>
-__device__ __forceinline__ void challenge(const uint64_t (&ct)[4], uint32_t *masks) {
+__device__ __forceinline__ void challenge(const uint64_t (&ct)[4], uint32_t *masks, bool store = true) {
     static_assert(N - TAU >= 192, "challenge positions i lie in mask words 6..7");
     uint32_t nz[8], neg[8];
 #pragma unroll
@@ -186,6 +188,7 @@ __device__ __forceinline__ void challenge(const uint64_t (&ct)[4], uint32_t *mas
         keccak_f1600(s);
         static_for<0, SHAKE256_RATE / 4>(bytes_of);
     }
+    if (!store) return;
     uint4 *o = reinterpret_cast<uint4 *>(masks);
     o[0] = make_uint4(nz[0], nz[1], nz[2], nz[3]);
     o[1] = make_uint4(nz[4], nz[5], nz[6], nz[7]);

@@ -278,8 +278,9 @@ int dil_server::enqueue(Lane &ln) {
         e = dil_sign_device_resident(ln.ctx, ks[ms], d_idx, ln.d_out, sigb, d_msg, d_moff, d_mlen, n,
                                      DIL_ENGINE_PERSISTENT, ln.st);
     } else if (ln.op == DIL_OP_VERIFY) {
-        e = dil_verify_device_resident(ks[ms], d_idx, reinterpret_cast<int32_t *>(ln.d_out), d_pay, sigb, d_msg,
-                                       d_moff, d_mlen, n, ln.d_work, ln.work_pitch, ln.st);
+        e = dil_verify_device_resident_ex(ks[ms], d_idx, reinterpret_cast<int32_t *>(ln.d_out), d_pay, sigb, d_msg,
+                                          d_moff, d_mlen, n, ln.d_work, ln.work_pitch,
+                                          ln.cls ? DIL_HASH_LONG_MESSAGES : 0, ln.st);
     } else {
         e = dil_keypair_device(mode, ln.d_out, pkb + skb, ln.d_out + pkb, pkb + skb, d_pay, 32, 0, n, ln.d_work,
                                ln.work_pitch, ln.st);
@@ -415,7 +416,11 @@ int dil_server_create(dil_server **out, const dil_server_config *cfg) {
         chk((int)hipMalloc((void **)&ln.d_in, in_bytes));
         chk((int)hipMalloc((void **)&ln.d_out, out_bytes));
         if (ln.work_pitch) chk((int)hipMalloc((void **)&ln.d_work, b * ln.work_pitch));
-        if (ln.op == DIL_OP_SIGN) chk(dil_sign_ctx_create_resident(&ln.ctx, ln.mode, b, b));
+        if (ln.op == DIL_OP_SIGN) {
+            chk(dil_sign_ctx_create_resident(&ln.ctx, ln.mode, b, b));
+            // long-message lanes hash each message with a 16-lane row
+            if (ln.ctx && ln.cls) chk(dil_sign_ctx_set_message_hashing(ln.ctx, DIL_HASH_LONG_MESSAGES));
+        }
     }
     if (e) {
         for (auto &ln : s->lanes) free_lane(ln);

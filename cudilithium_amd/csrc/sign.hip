@@ -151,11 +151,12 @@ template <class P>
 int sign_fixed_impl(const SignArgs &a, hipStream_t st) {
     if (a.n == 0) return 0;
     constexpr int KL = P::K * P::L, NP = P::L + 2 * P::K;
-    if (a.key_idx) {
-        // resident keys carry A, s1^, s2^, t0^ already: only mu / rhoprime
-        hipLaunchKernelGGL(sg_mu_kernel<P>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a);
+    auto mu = [&](hipStream_t s) -> int {
+        if (a.row_mu) return launch_sign_mu_row(P::MODE, a, s);
+        hipLaunchKernelGGL(sg_mu_kernel<P>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
         return (int)hipGetLastError();
-    }
+    };
+    if (a.key_idx) return mu(st);  // resident keys carry A, s1^, s2^, t0^ already: only mu / rhoprime
     AuxStreams *aux = aux_streams(st);
     if (!aux) return (int)hipErrorOutOfMemory;
     std::lock_guard<std::mutex> guard(aux->lock);
@@ -163,7 +164,7 @@ int sign_fixed_impl(const SignArgs &a, hipStream_t st) {
     if ((e = (int)hipEventRecord(aux->fork, st))) return e;
     if ((e = (int)hipStreamWaitEvent(aux->s[0], aux->fork, 0))) return e;
     if ((e = (int)hipStreamWaitEvent(aux->s[1], aux->fork, 0))) return e;
-    hipLaunchKernelGGL(sg_mu_kernel<P>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, aux->s[0], a);
+    if ((e = mu(aux->s[0]))) return e;
     hipLaunchKernelGGL(sg_prep_kernel<P>, dim3(default_grid_blocks(a.n * NP, SIGN_WAVES)), dim3(64 * SIGN_WAVES), 0,
                        aux->s[1], a);
     hipLaunchKernelGGL(sg_expand_kernel<P>, dim3((unsigned)((a.n * KL + 255) / 256)), dim3(256), 0, st, a);

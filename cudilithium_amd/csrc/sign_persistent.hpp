@@ -10,11 +10,11 @@
 //            slots o, o + U, o + 2U, ... with consecutive nonces, so freed
 //            slots turn into speculative nonces of unfinished items instead of
 //            idling at the end of the batch
-//   y w c z  the four attempt stages on the S slots (sign_stages.hpp), with
-//            the lane mapping each stage wants, separated by barriers
-//   commit   per item, the first successful slot in slot order is the
-//            smallest successful nonce -- the sequential specification's
-//            result -- and its signature is copied out
+//   y w c    the first three attempt stages on the S slots (sign_stages.hpp),
+//            with the lane mapping each stage wants, separated by barriers
+//   z        one wave per item: its slots in slot order = nonce order; the
+//            first that passes is the smallest successful nonce -- the
+//            sequential specification's result -- and writes the signature
 //
 // Workgroups never wait on each other (the queue is only read-modify-written
 // by atomicAdd), so there is no co-residency requirement and no deadlock: a
@@ -52,7 +52,6 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
     __shared__ uint32_t own_item[S], own_next[S];
     __shared__ int own_done[S];
     __shared__ uint32_t slot_own[S], slot_nonce[S];
-    __shared__ int slot_ok[S];
     __shared__ uint8_t w_unit[S];  // w work units: slot s, paired with s + u when s / u is even
     __shared__ int n_own, queue_empty, n_units;
     const int tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
@@ -149,37 +148,40 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
         __syncthreads();
         mark(2);
         __builtin_amdgcn_s_setprio(KECCAK_PRIO);
-        for (int s = tid; s < S; s += PERSIST_THREADS) stage_c<P>(a, own_item[slot_own[s]], slot_base(s), slot_sig(s));
+        // two lanes per slot (keccak_f1600_pair): S slots on 2 S lanes
+        for (int t = tid; t < 2 * S; t += PERSIST_THREADS) {
+            const int s = t >> 1;
+            stage_c_pair<P>(a, own_item[slot_own[s]], slot_base(s), slot_sig(s), (uint32_t)(t & 1));
+        }
         __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         mark(3);
-        for (int s = wave; s < S; s += NW) {
-            const bool ok = stage_z<P>(a, own_item[slot_own[s]], slot_base(s), slot_sig(s), x[wave], zt);
-            if (lane == 0) slot_ok[s] = ok ? 1 : 0;
-        }
-        __syncthreads();
-        mark(4);
-        // commit: slots of item o are o, o + u, ... in increasing nonce order
+        // z and commit, one wave per owned item: its slots o, o + u, ... carry
+        // increasing nonces, so they are checked in that order and the first
+        // that passes -- the sequential specification's result -- ends the
+        // item; later (speculative) slots of the item are skipped.  An attempt
+        // writes z and h straight into the item's signature; a rejected one
+        // leaves them partly written, and the accepted attempt rewrites every
+        // byte (c~ is copied from the slot where the c stage put it).
         for (int o = wave; o < u; o += NW) {
-            int win = -1;
-            for (int s = o; s < S; s += u)
-                if (slot_ok[s]) {
-                    win = s;
+            const size_t item = own_item[o];
+            uint8_t *dst = a.sig + item * a.sig_pitch;
+            for (int s = o; s < S; s += u) {
+                if (stage_z<P>(a, item, slot_base(s), dst, x[wave], zt)) {
+                    const uint8_t *src = slot_sig(s);
+                    if (lane < SEEDBYTES) dst[lane] = src[lane];
+                    if (lane == 0) {
+                        own_done[o] = 1;
+                        // attempts the sequential specification makes for this item
+                        *reinterpret_cast<uint32_t *>(a.rows + item * a.row_pitch + SignLayout<P>::ATTEMPTS) =
+                            slot_nonce[s] + 1;
+                    }
                     break;
-                }
-            if (win >= 0) {
-                const uint8_t *src = slot_sig(win);
-                uint8_t *dst = a.sig + (size_t)own_item[o] * a.sig_pitch;
-                for (int b = lane; b < Sizes<P>::SIG_BYTES; b += 64) dst[b] = src[b];
-                if (lane == 0) {
-                    own_done[o] = 1;
-                    // attempts the sequential specification makes for this item
-                    *reinterpret_cast<uint32_t *>(a.rows + (size_t)own_item[o] * a.row_pitch +
-                                                  SignLayout<P>::ATTEMPTS) = slot_nonce[win] + 1;
                 }
             }
         }
         __syncthreads();
+        mark(4);
         if (tid == 0) {
             int k = 0;
             for (int o = 0; o < u; ++o)

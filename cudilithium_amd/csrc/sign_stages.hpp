@@ -46,6 +46,9 @@ struct SignArgs {
     // rnd + i * rnd_pitch instead of SHAKE256(key || mu) (api.cu:143-148).
     const uint8_t *rnd;
     size_t rnd_pitch;
+    // Long messages: mu / rhoprime with one sponge per 16-lane row
+    // (sg_mu_row_kernel) instead of one lane per item.
+    int row_mu;
 };
 
 __device__ __forceinline__ size_t sk_index(const SignArgs &a, size_t i) {
@@ -276,6 +279,34 @@ __device__ __forceinline__ void stage_c(const SignArgs &a, size_t item, uint8_t 
 #pragma unroll
         for (int b = 0; b < 8; ++b) sig_out[8 * w + b] = (uint8_t)(ct[w] >> (8 * b));
     challenge<P::TAU>(ct, reinterpret_cast<uint32_t *>(slot + SL::CP));
+}
+
+// stage_c over a lane pair (keccak_f1600_pair): lanes 2j and 2j + 1 hash the
+// same slot, each holding one 32-bit half of every state word, so a wave
+// runs the challenge hash of 32 slots with a chain a third shorter than one
+// state per lane.  h = lane & 1; both lanes of the pair must call it.
+template <class P>
+__device__ __forceinline__ void stage_c_pair(const SignArgs &a, size_t item, uint8_t *slot, uint8_t *sig_out,
+                                             uint32_t h) {
+    using Z = Sizes<P>;
+    using SL = SlotLayout<P>;
+    const uint32_t *mu = reinterpret_cast<const uint32_t *>(a.rows + item * a.row_pitch + SignLayout<P>::MU);
+    const uint32_t *w1 = reinterpret_cast<const uint32_t *>(slot + SL::W1PK);
+    uint32_t s[25];
+    shake_absorb_pair<SHAKE256_RATE, Z::CHASH_IN_BYTES>(
+        s, [&](uint32_t w) { return w < 8 ? mu[2 * w + h] : w1[2 * (w - 8) + h]; }, h);
+    keccak_f1600_pair(s, h);
+    uint64_t ct[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) ct[w] = pair_word(s, w, h);
+    // c~: each lane of the pair writes its 4-byte halves (16 bytes)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const uint32_t v = s[w];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) sig_out[8 * w + 4 * h + b] = (uint8_t)(v >> (8 * b));
+    }
+    challenge<P::TAU>(ct, reinterpret_cast<uint32_t *>(slot + SL::CP), h == 0);
 }
 
 // Returns true (wave-uniform) when the attempt passes every check; then

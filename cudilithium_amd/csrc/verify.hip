@@ -68,30 +68,32 @@ __global__ __launch_bounds__(256) void vf_prep_kernel(VerifyArgs a) {
     if (bad) return;
 
     // tr = SHAKE256(pk)[0:32] (stored with a resident key); mu = SHAKE256(tr || m)[0:64]
-    // (verify.cu:130-144)
-    Lane64 s[25];
-    uint64_t tr[4];
-    if (key) {
+    // (verify.cu:130-144); already in the work row when vf_mu_row_kernel ran
+    if (!a.row_mu) {
+        Lane64 s[25];
+        uint64_t tr[4];
+        if (key) {
 #pragma unroll
-        for (int w = 0; w < 4; ++w) tr[w] = load_le64(key + KeyLayout<P>::TR + 8 * w);
-    } else {
-        shake_absorb<SHAKE256_RATE>(s, [&](uint32_t w, uint32_t nb) {
-            return nb == 8 ? load_le64(pk + 8 * w) : load_le_partial(pk + 8 * w, nb);
-        }, (uint32_t)Z::PK_BYTES);
-        squeeze_words<4>(s, tr);
+            for (int w = 0; w < 4; ++w) tr[w] = load_le64(key + KeyLayout<P>::TR + 8 * w);
+        } else {
+            shake_absorb<SHAKE256_RATE>(s, [&](uint32_t w, uint32_t nb) {
+                return nb == 8 ? load_le64(pk + 8 * w) : load_le_partial(pk + 8 * w, nb);
+            }, (uint32_t)Z::PK_BYTES);
+            squeeze_words<4>(s, tr);
+        }
+        const uint8_t *m = a.m.ptr(i);
+        const uint32_t mlen = a.m.len(i);
+        shake_absorb<SHAKE256_RATE>(s, [&](uint32_t w, uint32_t nb) -> uint64_t {
+            if (w < 4) return w == 0 ? tr[0] : (w == 1 ? tr[1] : (w == 2 ? tr[2] : tr[3]));
+            const uint8_t *p = m + 8 * (w - 4);
+            return nb == 8 ? load_le64(p) : load_le_partial(p, nb);
+        }, SEEDBYTES + mlen);
+        uint64_t mu[8];
+        squeeze_words<8>(s, mu);
+        uint64_t *hash = reinterpret_cast<uint64_t *>(work + VL::HASH);
+#pragma unroll
+        for (int w = 0; w < 8; ++w) hash[w] = mu[w];
     }
-    const uint8_t *m = a.m.ptr(i);
-    const uint32_t mlen = a.m.len(i);
-    shake_absorb<SHAKE256_RATE>(s, [&](uint32_t w, uint32_t nb) -> uint64_t {
-        if (w < 4) return w == 0 ? tr[0] : (w == 1 ? tr[1] : (w == 2 ? tr[2] : tr[3]));
-        const uint8_t *p = m + 8 * (w - 4);
-        return nb == 8 ? load_le64(p) : load_le_partial(p, nb);
-    }, SEEDBYTES + mlen);
-    uint64_t mu[8];
-    squeeze_words<8>(s, mu);
-    uint64_t *hash = reinterpret_cast<uint64_t *>(work + VL::HASH);
-#pragma unroll
-    for (int w = 0; w < 8; ++w) hash[w] = mu[w];
 
     uint64_t ct[4];
 #pragma unroll
@@ -104,10 +106,11 @@ struct VfMainCfg {
     static constexpr int G = P::MODE == 2 ? 16 : (P::MODE == 3 ? 8 : 4);
 };
 
-// RES: keys are resident (A and t1^ read from the store; no accumulator)
-template <class P, bool RES>
+// RES: keys are resident (A and t1^ read from the store; no accumulator);
+// G items per workgroup (the compiled default VfMainCfg<P>::G, or the value
+// a tiling chooses among those whose LDS fits in 160 KiB)
+template <class P, bool RES, int G>
 struct VfShared {
-    static constexpr int G = VfMainCfg<P>::G;
     ZetaLds zt;
     int32_t x[4][NTT_XBUF];
     int32_t zhat[G][P::L][N];
@@ -116,14 +119,13 @@ struct VfShared {
     int32_t fail[G];
 };
 
-template <class P, bool RES>
+template <class P, bool RES, int G>
 __global__ __launch_bounds__(256) void vf_main_kernel(VerifyArgs a) {
     using Z = Sizes<P>;
     using VL = VerifyLayout<P>;
     using KL = KeyLayout<P>;
-    constexpr int G = VfMainCfg<P>::G;
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    VfShared<P, RES> &sh = *reinterpret_cast<VfShared<P, RES> *>(smem_raw);
+    VfShared<P, RES, G> &sh = *reinterpret_cast<VfShared<P, RES, G> *>(smem_raw);
     const size_t item0 = (size_t)blockIdx.x * G;
     const int wave = threadIdx.x >> 6;
     zeta_lds_load(sh.zt);
@@ -256,27 +258,66 @@ __global__ __launch_bounds__(256) void vf_final_kernel(VerifyArgs a) {
     a.ret[i * a.ret_pitch] = verdict;
 }
 
-template <class P, bool RES>
-static int launch_vf_main(const VerifyArgs &a, hipStream_t st) {
-    constexpr int G = VfMainCfg<P>::G;
-    const size_t smem = sizeof(VfShared<P, RES>);
-    // the attribute is set once per kernel (a host call that need not be
-    // repeated per launch; it is not a stream operation)
-    static const hipError_t attr = hipFuncSetAttribute((const void *)vf_main_kernel<P, RES>,
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (attr != hipSuccess) return (int)attr;
-    hipLaunchKernelGGL((vf_main_kernel<P, RES>), dim3((unsigned)((a.n + G - 1) / G)), dim3(256), smem, st, a);
-    return 0;
+template <class P, int G>
+constexpr bool vf_fits() {
+    return sizeof(VfShared<P, false, G>) <= 160 * 1024;  // the resident form needs less
 }
 
-// ev (optional): four events recorded before vf_prep, between the launches
-// and after vf_final, for per-kernel timing on the launch stream
+template <class P, bool RES, int G>
+static int launch_vf_main_g(const VerifyArgs &a, hipStream_t st) {
+    if constexpr (!vf_fits<P, G>()) {
+        return -1;
+    } else {
+        const size_t smem = sizeof(VfShared<P, RES, G>);
+        // the attribute is set once per kernel (a host call that need not be
+        // repeated per launch; it is not a stream operation)
+        static const hipError_t attr = hipFuncSetAttribute((const void *)vf_main_kernel<P, RES, G>,
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (attr != hipSuccess) return (int)attr;
+        hipLaunchKernelGGL((vf_main_kernel<P, RES, G>), dim3((unsigned)((a.n + G - 1) / G)), dim3(256), smem, st, a);
+        return 0;
+    }
+}
+
+template <class P, bool RES>
+static int launch_vf_main(const VerifyArgs &a, hipStream_t st) {
+    switch (tiling_get(TILE_VERIFY_ITEMS, P::MODE, a.n)) {
+        case 2: return launch_vf_main_g<P, RES, 2>(a, st);
+        case 4: return launch_vf_main_g<P, RES, 4>(a, st);
+        case 8: return launch_vf_main_g<P, RES, 8>(a, st);
+        case 16: return launch_vf_main_g<P, RES, 16>(a, st);
+        default: return launch_vf_main_g<P, RES, VfMainCfg<P>::G>(a, st);
+    }
+}
+
+template <class P>
+static bool verify_items_ok(int g) {
+    switch (g) {
+        case 0: return true;
+        case 2: return vf_fits<P, 2>();
+        case 4: return vf_fits<P, 4>();
+        case 8: return vf_fits<P, 8>();
+        case 16: return vf_fits<P, 16>();
+        default: return false;
+    }
+}
+
+bool verify_items_supported(int mode, int g) {
+    switch (mode) {
+        case 2: return verify_items_ok<Params<2>>(g);
+        case 3: return verify_items_ok<Params<3>>(g);
+        case 5: return verify_items_ok<Params<5>>(g);
+        default: return false;
+    }
+}
+
 template <class P>
 static int verify_impl(const VerifyArgs &a, hipStream_t st, hipEvent_t *ev) {
     if (a.n == 0) return 0;
     const unsigned b1 = (unsigned)((a.n + 255) / 256);
     int e;
     if (ev && (e = (int)hipEventRecord(ev[0], st))) return e;
+    if (a.row_mu && (e = launch_verify_mu_row(P::MODE, a, st))) return e;
     hipLaunchKernelGGL(vf_prep_kernel<P>, dim3(b1), dim3(256), 0, st, a);
     if (ev && (e = (int)hipEventRecord(ev[1], st))) return e;
     e = a.key_idx ? launch_vf_main<P, true>(a, st) : launch_vf_main<P, false>(a, st);

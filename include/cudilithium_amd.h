@@ -218,6 +218,18 @@ int dil_verify_device(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig
                       const uint8_t *d_pk, size_t pk_pitch, const uint8_t *d_msg,
                       const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, uint8_t *d_work,
                       size_t work_pitch, void *stream);
+/* Message hashing of the verify / sign calls below: by default one lane
+ * absorbs a message serially (one SHAKE256 permutation per 136 bytes, the
+ * throughput form).  DIL_HASH_LONG_MESSAGES gives each message a 16-lane
+ * row of ten active lanes instead, which shortens the serial chain about
+ * three times at more instructions per message: for batches whose latency
+ * is set by long messages.  Results are identical. */
+#define DIL_HASH_LONG_MESSAGES 1
+int dil_verify_device_ex(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch,
+                         const uint8_t *d_pk, size_t pk_pitch, const uint8_t *d_msg,
+                         const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, uint8_t *d_work,
+                         size_t work_pitch, int flags, void *stream);
+
 /* dil_verify_device with its three launches (vf_prep, vf_main, vf_final)
  * timed by HIP events on `stream`; waits for the call and writes the three
  * durations in ms to kernel_ms3.  A measurement aid, same results. */
@@ -225,6 +237,22 @@ int dil_verify_device_timed(int mode, int32_t *d_ret, const uint8_t *d_sig, size
                             const uint8_t *d_pk, size_t pk_pitch, const uint8_t *d_msg,
                             const uint64_t *d_moff, const uint32_t *d_mlen, size_t n,
                             uint8_t *d_work, size_t work_pitch, float *kernel_ms3, void *stream);
+
+/* Run-time tilings of the batched kernels, as the autotuner
+ * (cudilithium_amd/autotune.py) records them: per kind, mode and batch
+ * bucket (0: up to 1,024 items, 1: 4,096, 2: 16,384, 3: more; -1 = all
+ * buckets) a value, 0 restoring the compiled default.  Kinds: keys per
+ * workgroup of keygen's main kernel and items per workgroup of verify's
+ * (2, 4, 8 or 16 where its LDS fits: defaults 16 / 8 / 4 for modes 2 / 3 / 5),
+ * wavefronts per workgroup of the standalone NTT (2, 4 or 8; default 4; mode
+ * ignored).  Process-wide; results never depend on them.  DIL_EINVAL for a
+ * value the kernels are not built for. */
+#define DIL_TILE_KEYGEN_ITEMS 0
+#define DIL_TILE_VERIFY_ITEMS 1
+#define DIL_TILE_NTT_WAVES 2
+int dil_set_kernel_tiling(int kind, int mode, int bucket, int value);
+/* The value in effect for a batch of n items (0: the compiled default). */
+int dil_get_kernel_tiling(int kind, int mode, size_t n);
 
 /* Signing context: owns the per-item rows (dil_sign_pool_row_bytes(mode, 0)
  * each), nslots attempt slots and the scheduler tables of either engine. */
@@ -273,6 +301,9 @@ int dil_sign_ctx_executed_attempts(dil_sign_ctx *ctx, uint64_t *count, void *str
  * c, z, commit) and the sums over workgroups of the last call are returned by
  * dil_sign_ctx_phase_cycles (synchronises `stream`).  Off by default. */
 int dil_sign_ctx_profile_phases(dil_sign_ctx *ctx, int enable);
+/* Message hashing of the context's sign calls: 0 or DIL_HASH_LONG_MESSAGES
+ * (mu and rhoprime with one sponge per 16-lane row). */
+int dil_sign_ctx_set_message_hashing(dil_sign_ctx *ctx, int flags);
 int dil_sign_ctx_phase_cycles(dil_sign_ctx *ctx, uint64_t *cycles6, void *stream);
 
 /* Resident keys (no counterpart in the reference, which re-derives A from
@@ -323,6 +354,11 @@ int dil_verify_device_resident(const dil_keystore *ks, const uint32_t *d_key_idx
                                const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_msg,
                                const uint64_t *d_moff, const uint32_t *d_mlen, size_t n,
                                uint8_t *d_work, size_t work_pitch, void *stream);
+/* The same with message-hashing flags (0 or DIL_HASH_LONG_MESSAGES). */
+int dil_verify_device_resident_ex(const dil_keystore *ks, const uint32_t *d_key_idx, int32_t *d_ret,
+                                  const uint8_t *d_sig, size_t sig_pitch, const uint8_t *d_msg,
+                                  const uint64_t *d_moff, const uint32_t *d_mlen, size_t n,
+                                  uint8_t *d_work, size_t work_pitch, int flags, void *stream);
 
 /* Randomized signing: the reference's DILITHIUM_RANDOMIZED_SIGNING build
  * (api.cu:143-148), where rhoprime is 64 random bytes per item instead of
