@@ -10,7 +10,9 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcudilithium_amd.so")
+# CUDILITHIUM_AMD_LIB: another build of the library (for example one compiled
+# with different tuning flags by build.build(out=..., extra_flags=...))
+LIB_PATH = os.environ.get("CUDILITHIUM_AMD_LIB") or os.path.join(HERE, "libcudilithium_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "cudilithium_amd.h")
 
 c_void_p = ctypes.c_void_p

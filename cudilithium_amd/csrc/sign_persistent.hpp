@@ -30,7 +30,12 @@
 
 namespace dil {
 
-constexpr int PERSIST_THREADS = 256;
+// threads per workgroup (4 waves; a build may choose 2 with -DDIL_PERSIST_THREADS=128,
+// then with 16 slots per workgroup so that the y stage's S * L sponges fit one pass)
+#ifndef DIL_PERSIST_THREADS
+#define DIL_PERSIST_THREADS 256
+#endif
+constexpr int PERSIST_THREADS = DIL_PERSIST_THREADS;
 constexpr uint32_t PERSIST_MAX_ROUNDS = 20000;
 // control words (uint32): 0 queue head, 1 status, 2 attempts executed.
 // Optional phase profile: PERSIST_PHASES uint64 cycle sums (assign, y, w, c,
@@ -38,7 +43,10 @@ constexpr uint32_t PERSIST_MAX_ROUNDS = 20000;
 constexpr int PERSIST_PHASES = 6;
 // wave priority (s_setprio) of the Keccak stages; 0 elsewhere.  Measured on
 // 16384-item batches: +2-3 % signatures/s for Dilithium3 over no priority.
-constexpr int KECCAK_PRIO = 2;
+#ifndef DIL_KECCAK_PRIO
+#define DIL_KECCAK_PRIO 2
+#endif
+constexpr int KECCAK_PRIO = DIL_KECCAK_PRIO;
 
 // WPE: waves per SIMD the register allocation must allow (512 / WPE VGPRs)
 // claim: at most this many items owned per workgroup (1..S); below S every
@@ -65,15 +73,20 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
         queue_empty = 0;
     }
     __syncthreads();
-    // optional phase profile: workgroup cycles per phase, summed over workgroups
+    // optional phase profile: workgroup cycles per phase, summed over
+    // workgroups; kept in LDS (only thread 0 uses it) so that it holds no
+    // registers through the loop
+    __shared__ unsigned long long tph[PERSIST_PHASES + 1];  // [PERSIST_PHASES]: last mark
     const bool prof = phase_sums != nullptr;
-    uint64_t tph[PERSIST_PHASES] = {0, 0, 0, 0, 0, 0};
-    uint64_t tlast = prof ? clock64() : 0;
+    if (prof && tid == 0) {
+        for (int p = 0; p < PERSIST_PHASES; ++p) tph[p] = 0;
+        tph[PERSIST_PHASES] = clock64();
+    }
     auto mark = [&](int ph) {
         if (prof && tid == 0) {
-            const uint64_t t = clock64();
-            tph[ph] += t - tlast;
-            tlast = t;
+            const unsigned long long t = clock64();
+            tph[ph] += t - tph[PERSIST_PHASES];
+            tph[PERSIST_PHASES] = t;
         }
     };
 
@@ -196,8 +209,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
         mark(5);
     }
     if (prof && tid == 0) {
-#pragma unroll
-        for (int p = 0; p < PERSIST_PHASES; ++p) atomicAdd(phase_sums + p, (unsigned long long)tph[p]);
+        for (int p = 0; p < PERSIST_PHASES; ++p) atomicAdd(phase_sums + p, tph[p]);
     }
 }
 
