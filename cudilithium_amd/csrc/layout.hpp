@@ -89,6 +89,12 @@ struct VerifyLayout {
     static constexpr size_t FLAG = HINT + 256;                        // int32 status
     static constexpr size_t M = FLAG + 256;
     static size_t row(size_t mlen) { return M + align256(mlen); }
+    // The verify kernels touch only [HASH, M) of a work row when signature,
+    // public key and message live elsewhere (the device-resident entry
+    // points): a compact work area of WORK_BYTES per item serves as rows based
+    // HASH bytes before it.
+    static constexpr size_t WORK_BEGIN = HASH;
+    static constexpr size_t WORK_BYTES = M - HASH;
     // reference formula (test_vectors.cu:149-166)
     static size_t ref_row(size_t mlen) {
         return align256(Z::SIG_BYTES) + align256(Z::PK_BYTES) + align256(SEEDBYTES + mlen) +

@@ -230,6 +230,20 @@ int dil_verify_device_ex(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t 
                          const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, uint8_t *d_work,
                          size_t work_pitch, int flags, void *stream);
 
+/* dil_verify_device that allocates nothing: no counterpart in the reference
+ * (whose callers pass pitched pools); for callers that cannot allocate device
+ * memory -- code under HIP-graph capture, serving loops that forbid hipMalloc.
+ * The work rows live in a fixed array of DIL_VERIFY_NOALLOC_SCRATCH_BYTES per
+ * GPU inside the library (held until the process ends), so n is limited to
+ * dil_verify_noalloc_max_items(mode) (8192 for mode 5, 9362 for modes 2 and
+ * 3); larger n gives DIL_ERANGE.  All calls on one GPU share that array:
+ * calls that overlap on one GPU race, so serialise them (one stream, or
+ * synchronise between calls).  Results equal dil_verify_device's. */
+#define DIL_VERIFY_NOALLOC_SCRATCH_BYTES (16u << 20)
+size_t dil_verify_noalloc_max_items(int mode);
+int dil_verify_device_noalloc(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch,
+                              const uint8_t *d_pk, size_t pk_pitch, const uint8_t *d_msg,
+                              const uint64_t *d_moff, const uint32_t *d_mlen, size_t n, void *stream);
 /* dil_verify_device with its three launches (vf_prep, vf_main, vf_final)
  * timed by HIP events on `stream`; waits for the call and writes the three
  * durations in ms to kernel_ms3.  A measurement aid, same results. */
