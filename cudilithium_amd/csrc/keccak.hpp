@@ -283,9 +283,14 @@ __device__ __forceinline__ void keccak_round_pair(uint32_t (&a)[25], uint32_t rc
     a[0] ^= rc_half;
 }
 
-// h: this lane's half (lane & 1); both lanes of the pair must be active
+// h: this lane's half (lane & 1); both lanes of the pair must be active.
+// Rounds unrolled DIL_PAIR_UNROLL at a time (default 2: the signing loop's
+// code size matters more than the round-constant loads).
+#ifndef DIL_PAIR_UNROLL
+#define DIL_PAIR_UNROLL 2
+#endif
 __device__ __forceinline__ void keccak_f1600_pair(uint32_t (&a)[25], uint32_t h) {
-#pragma unroll 2
+#pragma unroll DIL_PAIR_UNROLL
     for (int r = 0; r < 24; ++r) {
         const uint64_t rc = kKeccakRC[r];
         keccak_round_pair(a, h ? (uint32_t)(rc >> 32) : (uint32_t)rc);

@@ -113,12 +113,19 @@ __device__ __forceinline__ void keccak_round_row(uint32_t (&a)[5], const RowLane
     a[0] ^= rc_half & r.iota_mask;
 }
 
+// Fully unrolled: the round constants become immediates (a scalar load of
+// the table per round would be waited for together with the LDS traffic,
+// which shares its counter), and the short code of the row form keeps the
+// kernel small.
+template <int I = 0>
+__device__ __forceinline__ void keccak_rounds_row(uint32_t (&a)[5], const RowLane &r, uint32_t *lds) {
+    constexpr uint64_t rc = kKeccak.rc[I];
+    keccak_round_row(a, r, lds, r.h ? (uint32_t)(rc >> 32) : (uint32_t)rc);
+    if constexpr (I + 1 < 24) keccak_rounds_row<I + 1>(a, r, lds);
+}
+
 __device__ __forceinline__ void keccak_f1600_row(uint32_t (&a)[5], const RowLane &r, uint32_t *lds) {
-#pragma unroll 1
-    for (int i = 0; i < 24; ++i) {
-        const uint64_t rc = kKeccakRC[i];
-        keccak_round_row(a, r, lds, r.h ? (uint32_t)(rc >> 32) : (uint32_t)rc);
-    }
+    keccak_rounds_row(a, r, lds);
 }
 
 // Word index of (x, y) in the state; words >= RW are outside the rate.
