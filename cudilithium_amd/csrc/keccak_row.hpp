@@ -3,8 +3,11 @@
 // A message of L bytes costs floor(L / 136) + 1 dependent permutations in a
 // sponge, whatever the batch: a 64 KiB message is 482 of them.  With one
 // state per lane (keccak.hpp) each permutation is ~4,300 dependent VALU
-// instructions of one lane; here ten lanes share a state and the chain per
-// round is about a third as long:
+// instructions of one lane; here ten lanes share a state and a round is 54
+// VALU instructions and one LDS round trip.  A lone wavefront runs a chained
+// permutation in 3.45 us on MI355X, against 8.64 us with one state per lane
+// and 5.93 us with two lanes per state (keccak_f1600_pair); the price is
+// throughput, 2.1 G permutations/s for the whole card against 8.8 G:
 //
 //   lane g = 2 y + h of the row (g < 10) holds half h (h = 0: bits 0-31,
 //   h = 1: bits 32-63) of the five words (x, y), x = 0..4, of row y in five
@@ -13,8 +16,10 @@
 //   theta  column parities: XOR over the row's lanes of equal h with three
 //          DPP row rotations (by 2, 4, 8; the zero lanes pad the 16-lane row);
 //          rotl(C, 1) takes the other half from the partner lane (quad_perm)
-//   rho    per lane and word a funnel shift of the own half with the
-//          partner's, amount and operand order fixed per lane at start-up
+//   rho    per lane and word one funnel shift of the own half (high operand)
+//          with the partner's (low operand): for R > 32 and R = 0 the lane
+//          produces the pair's other half of the rotated word instead, which
+//          pi writes to that half's place, so no lane selects operands
 //   pi     (x, y) -> (y, 2x + 3y) moves words across rows: five LDS writes to
 //          per-lane addresses and one row read back (in-order LDS traffic of
 //          one wavefront); padding lanes write to a dummy word and read zeros
@@ -39,9 +44,11 @@ namespace dil {
 // write; the stride spreads the four states of a wave over different banks
 constexpr int ROW_LDS_WORDS = 72;
 
+// v ^ (v moved by a DPP row pattern): update_dpp with a zero old value
+// folds into one v_xor_b32_dpp
 template <int CTRL>
-__device__ __forceinline__ uint32_t dpp_mov(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+__device__ __forceinline__ uint32_t xor_dpp(uint32_t v) {
+    return v ^ (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
 }
 constexpr int DPP_ROW_ROR2 = 0x122, DPP_ROW_ROR4 = 0x124, DPP_ROW_ROR8 = 0x128;
 
@@ -49,8 +56,7 @@ struct RowLane {
     uint32_t h, y;        // half, row (y >= 5: padding lane)
     bool active;          // y < 5
     uint32_t sh[5];       // rho funnel-shift amount of word (x, y)
-    bool other_high[5];   // rho: the partner's half is the funnel's high operand
-    uint32_t waddr[5];    // pi destination of word x (index into the LDS array)
+    uint32_t waddr[5];    // pi destination of word x's rotated half (index into the LDS array)
     uint32_t raddr;       // this lane's row after pi (5 words)
     uint32_t iota_mask;   // all ones on the y = 0 lanes
 };
@@ -69,40 +75,39 @@ __device__ __forceinline__ RowLane row_lane_setup(uint32_t *lds, uint32_t wave_b
 #pragma unroll
     for (int x = 0; x < 5; ++x) {
         const int R = r.active ? kKeccak.rho[x + 5 * (int)r.y] : 0;
-        // rotl by R of the (own, other) halves: funnel(own, other, 32 - R) for
-        // 0 < R < 32, funnel(other, own, 64 - R) for R > 32 and R = 0
-        r.other_high[x] = R == 0 || R > 32;
-        r.sh[x] = (uint32_t)(r.other_high[x] ? (64 - R) & 31 : 32 - R);
-        r.waddr[x] = r.active ? st + r.h * 25 + ((2 * x + 3 * r.y) % 5) * 5 + r.y : st + 55;
+        // half h of rotl(w, R) = funnel(own, other, 32 - R) for 0 < R <= 32;
+        // for R > 32 and R = 0, rotl(w, R) = rotl(swap(w), R - 32) and the
+        // same funnel with shift (64 - R) mod 32 gives half 1 - h of it
+        const bool swapped = R == 0 || R > 32;
+        r.sh[x] = (uint32_t)(swapped ? (64 - R) & 31 : 32 - R);
+        r.waddr[x] = r.active ? st + (r.h ^ (uint32_t)swapped) * 25 + ((2 * x + 3 * r.y) % 5) * 5 + r.y : st + 55;
     }
     r.raddr = r.active ? st + r.h * 25 + r.y * 5 : st + 50;
     r.iota_mask = (r.active && r.y == 0) ? 0xFFFFFFFFu : 0u;
     return r;
 }
 
-__device__ __forceinline__ uint32_t row_xor_reduce(uint32_t v) {
-    v ^= dpp_mov<DPP_ROW_ROR2>(v);
-    v ^= dpp_mov<DPP_ROW_ROR4>(v);
-    v ^= dpp_mov<DPP_ROW_ROR8>(v);
-    return v;
-}
-
 __device__ __forceinline__ void keccak_round_row(uint32_t (&a)[5], const RowLane &r, uint32_t *lds,
                                                  uint32_t rc_half) {
+    // column parities: XOR over the row's lanes of equal h, the five columns
+    // interleaved so that no DPP read waits on the previous write
     uint32_t c[5], co[5];
 #pragma unroll
-    for (int x = 0; x < 5; ++x) c[x] = row_xor_reduce(a[x]);
+    for (int x = 0; x < 5; ++x) c[x] = xor_dpp<DPP_ROW_ROR2>(a[x]);
+#pragma unroll
+    for (int x = 0; x < 5; ++x) c[x] = xor_dpp<DPP_ROW_ROR4>(c[x]);
+#pragma unroll
+    for (int x = 0; x < 5; ++x) c[x] = xor_dpp<DPP_ROW_ROR8>(c[x]);
 #pragma unroll
     for (int x = 0; x < 5; ++x) co[x] = pair_other(c[x]);
 #pragma unroll
     for (int x = 0; x < 5; ++x)
         a[x] = bitop3<kXor3>(a[x], c[(x + 4) % 5], rotl_half<1>(c[(x + 1) % 5], co[(x + 1) % 5]));
+    uint32_t o[5];
 #pragma unroll
-    for (int x = 0; x < 5; ++x) {
-        const uint32_t o = pair_other(a[x]);
-        const uint32_t hi = r.other_high[x] ? o : a[x], lo = r.other_high[x] ? a[x] : o;
-        lds[r.waddr[x]] = funnel(hi, lo, r.sh[x]);
-    }
+    for (int x = 0; x < 5; ++x) o[x] = pair_other(a[x]);
+#pragma unroll
+    for (int x = 0; x < 5; ++x) lds[r.waddr[x]] = funnel(a[x], o[x], r.sh[x]);
     wave_lds_sync();
     uint32_t b[5];
 #pragma unroll
