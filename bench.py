@@ -13,7 +13,7 @@ a serving summary (resident keys, continuous batching: p50 / p99 latency
 at two offered loads) and the drop-in call's rate with host buffers.
 
     python bench.py [--steps K] [--warmup W]
-                    [--workload sign3|sign2|sign5|verify5|ntt|copy|serve|dropin|refcond]
+                    [--workload sign3|sign2|sign5|verify5|ntt|copy|serve|dropin|refcond|keccak]
 """
 from __future__ import annotations
 
@@ -555,7 +555,8 @@ def bench_serve(args):
     keys = workloads.config5_keys(n_keys)
     caps = tuple(len(keys[m][0]) for m in workloads.SERVE_MODES)
     srv = api.Server(budget, lanes=args.lanes, max_batch=args.max_batch, max_keys=caps,
-                     batch_window_us=args.window_us, long_message_bytes=args.long_bytes)
+                     batch_window_us=args.window_us, long_message_bytes=args.long_bytes,
+                     long_lanes=args.long_lanes)
     for m in workloads.SERVE_MODES:
         srv.stores[m].keygen(0, keys[m][1])
     st0 = srv.stats()
@@ -673,7 +674,8 @@ def bench_serve(args):
             "config": {"workload": f"serving_mix_{n_keys}_resident_keys_keygen20_sign50_verify30_msg_loguniform_32B-64KiB",
                        "requests_per_run": n, "hbm_budget_bytes": budget, "lanes": args.lanes,
                        "max_batch": args.max_batch, "batch_window_us": args.window_us,
-                       "long_message_bytes": args.long_bytes, "batches": int(st.batches)},
+                       "long_message_bytes": args.long_bytes, "long_lanes": args.long_lanes or args.lanes,
+                       "batches": int(st.batches)},
             "saturation": sat, "loads": runs, "roofline": None, "cpu_baseline": None}
 
 
@@ -705,6 +707,35 @@ def bench_copy(args):
             "cpu_baseline": None}
 
 
+def bench_keccak(args):
+    """Chained Keccak-f[1600] permutations in the three state layouts (one
+    state per lane / per two lanes / per 16-lane row): latency of one wave
+    alone, and the whole card's throughput (2,048 workgroups of 256 threads)."""
+    import ctypes
+    from cudilithium_amd import api
+    L = api.L()
+    api._lib.register("dil_keccak_microbench", ctypes.c_int,
+                      [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)])
+    nperm = max(50, args.steps * 20)
+    forms = {}
+    for form, name in enumerate(("lane", "pair", "row")):
+        r = {}
+        for label, blocks, threads in (("one_wave", 1, 64), ("card", 2048, 256)):
+            us, rate = ctypes.c_double(), ctypes.c_double()
+            api._lib.check(L.dil_keccak_microbench(form, blocks, threads, nperm, ctypes.byref(us), ctypes.byref(rate)),
+                           "keccak_microbench")
+            r[label] = {"us_per_chained_perm": us.value, "perms_per_s": rate.value}
+        forms[name] = r
+        log(f"[bench] keccak {name}: one wave {r['one_wave']['us_per_chained_perm']:.2f} us per permutation; "
+            f"card {r['card']['perms_per_s'] / 1e9:.2f} G permutations/s")
+    return {"metric": "Keccak-f[1600] permutation latency / throughput", "value": forms["row"]["one_wave"]
+            ["us_per_chained_perm"], "unit": "us per chained permutation (row form, one wave)",
+            "higher_is_better": False, "ms_per_step": None, "dtype": "u32", "parity": None,
+            "config": {"workload": f"keccak_chains_{nperm}_permutations", "forms": forms},
+            "roofline": None, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -730,6 +761,7 @@ def main():
     ap.add_argument("--max-batch", type=int, default=4096, help="serve: requests per batch")
     ap.add_argument("--window-us", type=int, default=0, help="serve: batching window")
     ap.add_argument("--long-bytes", type=int, default=4096, help="serve: messages above this get lanes of their own")
+    ap.add_argument("--long-lanes", type=int, default=0, help="serve: lanes per long-message group (0: --lanes)")
     ap.add_argument("--fractions", type=float, nargs="*", default=[0.3, 0.6, 0.9],
                     help="serve: offered loads as fractions of the burst saturation")
     ap.add_argument("--rates", type=float, nargs="*", default=[25000, 50000, 100000],
@@ -804,6 +836,8 @@ def main():
         res = bench_ntt(args)
     elif w == "copy":
         res = bench_copy(args)
+    elif w == "keccak":
+        res = bench_keccak(args)
     elif w == "serve":
         res = bench_serve(args)
     elif w == "refcond":
@@ -813,7 +847,8 @@ def main():
                     "config": {"workload": res["workload"]}, "roofline": None, "cpu_baseline": None})
     else:
         raise SystemExit(f"unknown workload {w}")
-    res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+    res.setdefault("higher_is_better", True)
+    res.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                 "scaling": "weak", "vs_baseline": None, "data": "synthetic"})
     print(json.dumps(res), flush=True)
 

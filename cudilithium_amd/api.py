@@ -589,7 +589,7 @@ OP_KEYGEN, OP_SIGN, OP_VERIFY = 0, 1, 2
 class ServerConfig(ctypes.Structure):  # dil_server_config
     _fields_ = [("hbm_budget_bytes", c_size_t), ("lanes", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("max_keys", ctypes.c_uint32 * 3), ("batch_window_us", ctypes.c_uint32),
-                ("long_message_bytes", ctypes.c_uint32)]
+                ("long_message_bytes", ctypes.c_uint32), ("long_lanes", ctypes.c_uint32)]
 
 
 class Request(ctypes.Structure):  # dil_request
@@ -639,11 +639,11 @@ class Server:
     keeps alive until done (Server.requests builds such arrays)."""
 
     def __init__(self, hbm_budget_bytes: int, lanes: int = 1, max_batch: int = 4096, max_keys=(0, 0, 0),
-                 batch_window_us: int = 0, long_message_bytes: int = 4096):
+                 batch_window_us: int = 0, long_message_bytes: int = 4096, long_lanes: int = 0):
         L()
         _register_server()
         cfg = ServerConfig(hbm_budget_bytes, lanes, max_batch, (ctypes.c_uint32 * 3)(*max_keys), batch_window_us,
-                           long_message_bytes)
+                           long_message_bytes, long_lanes)
         p = c_void_p()
         _lib.check(L().dil_server_create(ctypes.byref(p), ctypes.byref(cfg)), "server_create")
         self.handle = p.value

@@ -159,6 +159,13 @@ int dil_valu_microbench(int op, uint32_t iters, double *lane_ops_per_s) {
     return valu_microbench(op, iters, lane_ops_per_s);
 }
 
+int dil_keccak_microbench(int form, uint32_t blocks, uint32_t threads, uint32_t nperm, double *us_per_perm,
+                          double *perms_per_s) {
+    if (!us_per_perm || !perms_per_s || blocks > (1u << 20)) return DIL_EINVAL;
+    const int e = keccak_microbench(form, blocks, threads, nperm, us_per_perm, perms_per_s);
+    return e == -1 ? DIL_EINVAL : e;
+}
+
 int dil_keccak_f1600_batch(uint64_t *d_states, size_t n, void *stream) {
     if (n && !d_states) return DIL_EINVAL;
     return launch_keccak(d_states, n, S(stream));

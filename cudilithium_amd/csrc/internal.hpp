@@ -52,5 +52,7 @@ int launch_ntt(int kind, int32_t *c, const int32_t *a, const int32_t *b, size_t 
                int blocks, hipStream_t stream);
 int launch_keccak(uint64_t *st, size_t n, hipStream_t stream);
 int valu_microbench(int op, uint32_t iters, double *ops_per_s);
+int keccak_microbench(int form, uint32_t blocks, uint32_t threads, uint32_t nperm, double *us_per_perm,
+                      double *perms_per_s);
 
 }  // namespace dil

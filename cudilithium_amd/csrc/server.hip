@@ -345,7 +345,8 @@ extern "C" {
 uint64_t dil_monotonic_ns(void) { return mono_ns(); }
 
 int dil_server_create(dil_server **out, const dil_server_config *cfg) {
-    if (!out || !cfg || cfg->lanes == 0 || cfg->lanes > 16 || cfg->max_batch == 0 || cfg->max_batch > 65536)
+    if (!out || !cfg || cfg->lanes == 0 || cfg->lanes > 16 || cfg->long_lanes > 16 || cfg->max_batch == 0 ||
+        cfg->max_batch > 65536)
         return DIL_EINVAL;
     dil_server *s = new dil_server();
     s->cfg = *cfg;
@@ -356,11 +357,11 @@ int dil_server_create(dil_server **out, const dil_server_config *cfg) {
     // the message areas of the sign and verify lanes
     s->long_bytes = cfg->long_message_bytes;
     std::vector<std::pair<int, int>> kinds;  // (group, class) of each lane
+    const uint32_t long_lanes = cfg->long_lanes ? cfg->long_lanes : cfg->lanes;
     for (uint32_t k = 0; k < cfg->lanes; ++k)
-        for (int g = 0; g < NGROUPS; ++g) {
-            kinds.push_back({g, 0});
-            if (s->long_bytes && g >= 3) kinds.push_back({g, 1});
-        }
+        for (int g = 0; g < NGROUPS; ++g) kinds.push_back({g, 0});
+    for (uint32_t k = 0; s->long_bytes && k < long_lanes; ++k)
+        for (int g = 3; g < NGROUPS; ++g) kinds.push_back({g, 1});
     s->lanes.resize(kinds.size());
     size_t fixed = 0, msg_lanes = 0, short_msg = 0;
     for (size_t i = 0; i < s->lanes.size(); ++i) {

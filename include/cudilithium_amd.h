@@ -222,7 +222,7 @@ int dil_verify_device(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig
  * absorbs a message serially (one SHAKE256 permutation per 136 bytes, the
  * throughput form).  DIL_HASH_LONG_MESSAGES gives each message a 16-lane
  * row of ten active lanes instead, which shortens the serial chain about
- * three times at more instructions per message: for batches whose latency
+ * 2.5 times at more instructions per message: for batches whose latency
  * is set by long messages.  Results are identical. */
 #define DIL_HASH_LONG_MESSAGES 1
 int dil_verify_device_ex(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t sig_pitch,
@@ -434,6 +434,8 @@ typedef struct dil_server_config {
   uint32_t batch_window_us; /* hold a partial batch this long for more requests */
   uint32_t long_message_bytes; /* sign / verify messages above this (bytes) get
                                   lanes of their own (0: no split) */
+  uint32_t long_lanes;      /* lanes per long-message (op, mode) group (0..16;
+                               0: `lanes`) */
 } dil_server_config;
 
 /* One request.  The caller owns it and every buffer it points to until
@@ -515,6 +517,14 @@ int dil_keccak_f1600_batch(uint64_t *d_states, size_t n, void *stream);
  * 3 mul_hi_i32, 4 fma_f64, 5 alignbit_b32, 6 one (cvt_f64_i32, mul_f64,
  * cvt_i32_f64, add) sequence, 7 mul_u32_u24. */
 int dil_valu_microbench(int op, uint32_t iters, double *lane_ops_per_s);
+
+/* Latency and throughput of chained Keccak-f[1600] permutations in the
+ * engine's three state layouts (form 0: one state per lane, 1: two lanes per
+ * state, 2: one state per 16-lane row): `blocks` workgroups of `threads`
+ * (64 or 256) threads each run `nperm` dependent permutations (synchronous).
+ * Returns microseconds per chained permutation and permutations per second. */
+int dil_keccak_microbench(int form, uint32_t blocks, uint32_t threads, uint32_t nperm, double *us_per_perm,
+                          double *perms_per_s);
 
 /* ---------------------------------------------------------------------- */
 /* Runtime helpers                                                        */
