@@ -63,6 +63,7 @@ def build(verbose: bool = True, out: str | None = None, extra_flags=()) -> str:
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, build_dir, tuple(extra_flags)), srcs))
+    os.makedirs(os.path.dirname(os.path.abspath(lib)), exist_ok=True)
     tmp = lib + ".tmp"
     r = subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs],
                        capture_output=True, text=True)
