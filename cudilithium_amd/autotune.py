@@ -202,6 +202,7 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--occupancy", type=int, nargs="*", default=list(OCCUPANCY))
     ap.add_argument("--write", action="store_true", help="record the best tiling in tuning.json")
+    ap.add_argument("--cache", default=None, help="tuning file to record in (default: the package's tuning.json)")
     args = ap.parse_args()
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
     if args.op == "sign":
@@ -215,7 +216,7 @@ def main():
         key = f"{args.op}{args.mode if args.op != 'ntt' else ''}:{bucket(args.batch)}"
     print(json.dumps({"key": key, "best": entry}))
     if args.write:
-        write_entry(key, entry)
+        write_entry(key, entry, args.cache)
 
 
 if __name__ == "__main__":

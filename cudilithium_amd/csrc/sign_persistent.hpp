@@ -57,6 +57,9 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
     SignArgs a, SlotRef sr, uint32_t *head, int claim, unsigned long long *phase_sums) {
     __shared__ ZetaLds zt;
     __shared__ int32_t x[PERSIST_THREADS / 64][NTT_XBUF];
+    // L = 7: the paired w stage's transformed y vectors (56 KiB per workgroup)
+    constexpr bool W_Y_LDS = P::L > 5;
+    __shared__ int32_t ylds[W_Y_LDS ? PERSIST_THREADS / 64 : 1][W_Y_LDS ? 2 * P::L * N : 1];
     __shared__ uint32_t own_item[S], own_next[S];
     __shared__ int own_done[S];
     __shared__ uint32_t slot_own[S], slot_nonce[S];
@@ -146,15 +149,10 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
         for (int t = wave; t < n_units; t += NW) {
             const int s = w_unit[t];
             const size_t item = own_item[slot_own[s]];
-            if constexpr (P::L <= 5) {
-                if (s + u < S) {
-                    uint8_t *const pair[2] = {slot_base(s), slot_base(s + u)};
-                    stage_w_multi<P, 2>(a, item, pair, x[wave], zt);
-                    continue;
-                }
-            } else if (s + u < S) {
-                // L = 7: two y vectors and two A rows exceed the register budget
-                stage_w<P>(a, item, slot_base(s + u), x[wave], zt);
+            if (s + u < S) {
+                uint8_t *const pair[2] = {slot_base(s), slot_base(s + u)};
+                stage_w_multi<P, 2>(a, item, pair, x[wave], zt, ylds[W_Y_LDS ? wave : 0]);
+                continue;
             }
             stage_w<P>(a, item, slot_base(s), x[wave], zt);
         }

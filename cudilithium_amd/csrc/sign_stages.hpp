@@ -196,24 +196,35 @@ __device__ __forceinline__ void load_y_A(int32_t (&v)[4], const uint8_t *slot, i
 // w for M attempts of the same item (slots[0..M)) in one wave: every A row
 // is loaded once for all M products, and the M transforms give the wave
 // independent work to overlap with the loads.
+// ylds: with M > 1 and L = 7 the M x 7 transformed y polynomials live in
+// this wave's LDS area (M * L * N words) instead of registers, where they
+// would not fit beside the A rows and the INTT
 template <class P, int M>
 __device__ __forceinline__ void stage_w_multi(const SignArgs &a, size_t item, uint8_t *const (&slots)[M], int32_t *x,
-                                              const ZetaLds &zt) {
+                                              const ZetaLds &zt, int32_t *ylds = nullptr) {
     using Z = Sizes<P>;
     using SL = SlotLayout<P>;
+    constexpr bool Y_LDS = M > 1 && P::L > 5;
     const ItemView<P> it(a, item);
     // A rows are streamed from HBM with the next row's loads issued before
     // the current row's products and INTTs (software prefetch)
     int32_t an[P::L][4];
 #pragma unroll
     for (int l = 0; l < P::L; ++l) wave_load_D(an[l], it.mat + l * N);
-    int32_t yh[M][P::L][4];
+    int32_t yh[Y_LDS ? 1 : M][Y_LDS ? 1 : P::L][4];
 #pragma unroll
     for (int j = 0; j < M; ++j)
 #pragma unroll
         for (int l = 0; l < P::L; ++l) {
-            load_y_A<P>(yh[j][l], slots[j], l);
-            ntt_wave(yh[j][l], x, zt);
+            int32_t t[4];
+            load_y_A<P>(t, slots[j], l);
+            ntt_wave(t, x, zt);
+            if constexpr (Y_LDS) {
+                wave_store_D(ylds + (j * P::L + l) * N, t);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) yh[j][l][r] = t[r];
+            }
         }
     for (int k = 0; k < P::K; ++k) {
         int32_t acc[M][4];
@@ -231,9 +242,17 @@ __device__ __forceinline__ void stage_w_multi(const SignArgs &a, size_t item, ui
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[j][r] = 0;
 #pragma unroll
-            for (int l = 0; l < P::L; ++l)
+            for (int l = 0; l < P::L; ++l) {
+                if constexpr (Y_LDS) {
+                    int32_t yv[4];
+                    wave_load_D(yv, ylds + (j * P::L + l) * N);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[j][r] += mont_mul(ac[l][r], yh[j][l][r]);
+                    for (int r = 0; r < 4; ++r) acc[j][r] += mont_mul(ac[l][r], yv[r]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[j][r] += mont_mul(ac[l][r], yh[j][l][r]);
+                }
+            }
         }
 #pragma unroll
         for (int j = 0; j < M; ++j) {
