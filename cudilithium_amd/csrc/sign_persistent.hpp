@@ -58,7 +58,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
     __shared__ ZetaLds zt;
     __shared__ int32_t x[PERSIST_THREADS / 64][NTT_XBUF];
     // L = 7: the paired w stage's transformed y vectors (56 KiB per workgroup)
-    constexpr bool W_Y_LDS = P::L > 5;
+    constexpr bool W_Y_LDS = P::L >= DIL_W_Y_LDS_MIN_L;
     __shared__ int32_t ylds[W_Y_LDS ? PERSIST_THREADS / 64 : 1][W_Y_LDS ? 2 * P::L * N : 1];
     __shared__ uint32_t own_item[S], own_next[S];
     __shared__ int own_done[S];

@@ -204,7 +204,12 @@ __device__ __forceinline__ void stage_w_multi(const SignArgs &a, size_t item, ui
                                               const ZetaLds &zt, int32_t *ylds = nullptr) {
     using Z = Sizes<P>;
     using SL = SlotLayout<P>;
-    constexpr bool Y_LDS = M > 1 && P::L > 5;
+// smallest L whose paired y vectors go to LDS (for L = 4 and 5 moving them
+// there freed 10-27 VGPRs and changed nothing measurable)
+#ifndef DIL_W_Y_LDS_MIN_L
+#define DIL_W_Y_LDS_MIN_L 6
+#endif
+    constexpr bool Y_LDS = M > 1 && P::L >= DIL_W_Y_LDS_MIN_L;
     const ItemView<P> it(a, item);
     // A rows are streamed from HBM with the next row's loads issued before
     // the current row's products and INTTs (software prefetch)
