@@ -37,12 +37,13 @@ PEAK_HBM_GBS = 8000.0
 # WRITE_SIZE passes scaled by a 1 GiB copy's readings (read = 2 x FETCH_SIZE,
 # written = WRITE_SIZE); profiles/sign3_16384_hbm_valu_counters.md.
 TRAFFIC_BYTES = {
-    "dilithium3_sign_16384_msg_loguniform_32B-4KiB": (2 * 1763618 + 1343747) * 1024,
-    # vf_prep + vf_main + vf_final per call
-    "dilithium5_verify_65536_msg_1KiB_10pct_bitflips": (2 * (142288 + 246351 + 46122) + (77760 + 65462 + 256)) * 1024,
+    # loop kernel per launch: 2 x FETCH_SIZE + WRITE_SIZE KiB (profiles/sign3_16384_hbm_valu_counters.md)
+    "dilithium3_sign_16384_msg_loguniform_32B-4KiB": (2 * 1520421 + 1264810) * 1024,
+    # vf_prep + vf_main + vf_final per call (profiles/verify5_65536_kernel_stats.md)
+    "dilithium5_verify_65536_msg_1KiB_10pct_bitflips": (2 * (142240 + 247481 + 46117) + (77063 + 65462 + 256)) * 1024,
     "ntt_pipeline_65536x256": (2 * 65576 + 65536) * 1024,
-    # vf_main alone (profiles/verify5_65536_kernel_stats.md)
-    "dilithium5_verify_65536_msg_1KiB_10pct_bitflips_vf_main": (2 * 246351 + 65462) * 1024,
+    # vf_main alone
+    "dilithium5_verify_65536_msg_1KiB_10pct_bitflips_vf_main": (2 * 247481 + 65462) * 1024,
 }
 
 

@@ -89,10 +89,17 @@ def apply_cache(L, path: str | None = None) -> int:
     return count
 
 
-def _time(fn, reps: int) -> float:
+def _time(fn, reps: int, min_seconds: float = 0.05) -> float:
+    """Mean time per call over at least `reps` calls and `min_seconds` (a
+    ~50 us kernel timed over 3 calls is mostly launch and clock noise)."""
     from .device import synchronize
     fn()
     synchronize()
+    t0 = time.perf_counter()
+    fn()
+    synchronize()
+    one = max(time.perf_counter() - t0, 1e-6)
+    reps = max(reps, min(1000, int(min_seconds / one) + 1))
     t0 = time.perf_counter()
     for _ in range(reps):
         fn()
