@@ -55,3 +55,16 @@ def test_dropin_sign_rejects_pools_not_16_byte_aligned():
         rc = L.dil_crypto_sign_signature(mode, host, 3293, ctypes.byref(siglen), host, mlen, mlen, host,
                                          fake, sign_pitch, fake, temp_pitch, ctypes.byref(lut), 2048, batch, None)
         assert rc == -1, rc  # DIL_EINVAL
+
+
+def test_keccak_microbench_rejects_bad_arguments():
+    """Argument checks run before any device call (DIL_EINVAL = -1)."""
+    L = _lib.load()
+    fn = L.dil_keccak_microbench
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    us, rate = ctypes.c_double(), ctypes.c_double()
+    for form, blocks, threads, nperm in ((3, 1, 64, 1), (0, 1, 128, 1), (0, 0, 64, 1), (0, 1, 64, 0)):
+        assert fn(form, blocks, threads, nperm, ctypes.byref(us), ctypes.byref(rate)) == -1
+    assert fn(0, 1, 64, 1, None, ctypes.byref(rate)) == -1

@@ -56,3 +56,20 @@ def test_pipeline_full_config_size_properties():
     idx = np.random.default_rng(5).choice(npoly, 512, replace=False)
     assert np.array_equal(got[idx], oracle.ntt_pipeline(a[idx], b[idx]))
     assert np.all(np.abs(got) < Q)
+
+
+def test_keccak_microbench_runs_every_layout():
+    """dil_keccak_microbench: chained permutations in the lane / pair / row
+    layouts report a positive latency and rate (the layouts' results are
+    pinned end to end by the signing and long-message parity tests)."""
+    import ctypes
+    from cudilithium_amd import api
+    L = api.L()
+    api._lib.register("dil_keccak_microbench", ctypes.c_int,
+                      [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)])
+    for form in (0, 1, 2):
+        for blocks, threads in ((1, 64), (8, 256)):
+            us, rate = ctypes.c_double(), ctypes.c_double()
+            assert L.dil_keccak_microbench(form, blocks, threads, 8, ctypes.byref(us), ctypes.byref(rate)) == 0
+            assert 0 < us.value < 1e4 and rate.value > 0

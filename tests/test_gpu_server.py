@@ -12,10 +12,10 @@ pytestmark = pytest.mark.gpu
 KEYS = 8
 
 
-def _server(budget=384 << 20, lanes=1, max_batch=128, window=0, long_bytes=4096):
+def _server(budget=384 << 20, lanes=1, max_batch=128, window=0, long_bytes=4096, long_lanes=0):
     from cudilithium_amd import api
     srv = api.Server(budget, lanes=lanes, max_batch=max_batch, max_keys=(KEYS, KEYS, KEYS), batch_window_us=window,
-                     long_message_bytes=long_bytes)
+                     long_message_bytes=long_bytes, long_lanes=long_lanes)
     keys = {}
     for m in (2, 3, 5):
         seeds = np.random.default_rng(70 + m).integers(0, 256, (KEYS, 32), dtype=np.uint8)
@@ -161,13 +161,13 @@ def test_server_admission_errors():
     srv.close()
 
 
-@pytest.mark.parametrize("long_bytes", [512, 0])
-def test_server_message_classes(long_bytes):
+@pytest.mark.parametrize("long_bytes,long_lanes", [(512, 0), (512, 2), (0, 0)])
+def test_server_message_classes(long_bytes, long_lanes):
     """Messages above long_message_bytes go to lanes of their own (512: most
-    of this mix), or every message shares one lane per group (0); results
-    match the oracle either way."""
+    of this mix; one or two lanes per long-message group), or every message
+    shares one lane per group (0); results match the oracle either way."""
     from cudilithium_amd import api
-    srv, keys = _server(budget=256 << 20, lanes=1, max_batch=64, long_bytes=long_bytes)
+    srv, keys = _server(budget=256 << 20, lanes=1, max_batch=64, long_bytes=long_bytes, long_lanes=long_lanes)
     n = 160
     ops, modes, kidx, msgs, seeds, sigs, _ = _mix(n, keys, 11 + long_bytes, max_mlen=3000)
     exp = _expected(ops, modes, kidx, msgs, seeds, sigs, keys)
