@@ -51,6 +51,8 @@ bool ntt_waves_supported(int w);
 int launch_ntt(int kind, int32_t *c, const int32_t *a, const int32_t *b, size_t npoly, size_t pitch_words,
                int blocks, hipStream_t stream);
 int launch_keccak(uint64_t *st, size_t n, hipStream_t stream);
+// a stream with a hardware queue of its own (all-CU mask)
+int create_queue_stream(hipStream_t *st);
 int valu_microbench(int op, uint32_t iters, double *ops_per_s);
 int keccak_microbench(int form, uint32_t blocks, uint32_t threads, uint32_t nperm, double *us_per_perm,
                       double *perms_per_s);

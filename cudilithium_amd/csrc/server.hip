@@ -26,7 +26,7 @@
 // memory -- every lane's buffers and signing context and the message areas
 // -- is carved from the budget at creation, so admission never allocates; a
 // message larger than its group's message area is rejected (DIL_ERANGE).
-// Every lane's stream gets a hardware queue of its own (create_lane_stream):
+// Every lane's stream gets a hardware queue of its own (create_queue_stream):
 // streams that share a queue serialise, and measured with plain streams a
 // short message's batch waited behind a long message's hash.  For the same
 // reason the staging copies are kernels on the lane's stream that read and
@@ -50,6 +50,7 @@
 #include <vector>
 
 #include "../../include/cudilithium_amd.h"
+#include "internal.hpp"
 #include "layout.hpp"
 #include "params.hpp"
 
@@ -118,20 +119,6 @@ struct Lane {
     std::condition_variable *cv = nullptr;  // woken when its queue gets work
 };
 
-// A stream with a hardware queue of its own.  The HIP runtime may map several
-// streams created while the device is idle onto one hardware queue, where
-// their work serialises (a short message's batch then waits behind a long
-// message's hash); a stream created with a CU mask always gets its own
-// queue, and the mask here enables every CU, so nothing else changes.
-int create_lane_stream(hipStream_t *st) {
-    int cus = 0, dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        return (int)hipStreamCreateWithFlags(st, hipStreamNonBlocking);
-    std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0xFFFFFFFFu);
-    if (cus % 32) mask.back() = (1u << (cus % 32)) - 1u;
-    return (int)hipExtStreamCreateWithCUMask(st, (uint32_t)mask.size(), mask.data());
-}
 
 // Copies n16 16-byte words (both pointers 16-byte aligned); one side is a
 // pinned host buffer's device view.  Grid-stride, so any grid size covers n16.
@@ -405,7 +392,7 @@ int dil_server_create(dil_server **out, const dil_server_config *cfg) {
                      out_bytes = al256(b * out_item_bytes(ln.op, ln.mode)) + 256;
         s->lane_bytes_max = std::max(s->lane_bytes_max, in_bytes + out_bytes + b * ln.work_pitch +
                                                             (ln.op == DIL_OP_SIGN ? sign_ctx_bytes(ln.mode, b) : 0));
-        chk(create_lane_stream(&ln.st));
+        chk(create_queue_stream(&ln.st));
         chk((int)hipEventCreateWithFlags(&ln.ev, hipEventBlockingSync));
         chk((int)hipEventCreate(&ln.ev0));
         // coherent: the copy kernels' host writes are visible once the lane's event completes

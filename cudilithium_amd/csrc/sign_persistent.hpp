@@ -57,8 +57,11 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
     SignArgs a, SlotRef sr, uint32_t *head, int claim, unsigned long long *phase_sums) {
     __shared__ ZetaLds zt;
     __shared__ int32_t x[PERSIST_THREADS / 64][NTT_XBUF];
-    // L = 7: the paired w stage's transformed y vectors (56 KiB per workgroup)
-    constexpr bool W_Y_LDS = P::L >= DIL_W_Y_LDS_MIN_L;
+    // L = 7: the paired w stage's transformed y vectors (56 KiB per workgroup,
+    // which leaves room for two workgroups per CU: with more waves per SIMD
+    // the w stage does not pair for L = 7)
+    constexpr bool W_Y_LDS = P::L >= DIL_W_Y_LDS_MIN_L && WPE <= 2;
+    constexpr bool W_PAIRS = P::L < DIL_W_Y_LDS_MIN_L || W_Y_LDS;
     __shared__ int32_t ylds[W_Y_LDS ? PERSIST_THREADS / 64 : 1][W_Y_LDS ? 2 * P::L * N : 1];
     __shared__ uint32_t own_item[S], own_next[S];
     __shared__ int own_done[S];
@@ -150,9 +153,13 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
             const int s = w_unit[t];
             const size_t item = own_item[slot_own[s]];
             if (s + u < S) {
-                uint8_t *const pair[2] = {slot_base(s), slot_base(s + u)};
-                stage_w_multi<P, 2>(a, item, pair, x[wave], zt, ylds[W_Y_LDS ? wave : 0]);
-                continue;
+                if constexpr (W_PAIRS) {
+                    uint8_t *const pair[2] = {slot_base(s), slot_base(s + u)};
+                    stage_w_multi<P, 2>(a, item, pair, x[wave], zt, ylds[W_Y_LDS ? wave : 0]);
+                    continue;
+                } else {
+                    stage_w<P>(a, item, slot_base(s + u), x[wave], zt);
+                }
             }
             stage_w<P>(a, item, slot_base(s), x[wave], zt);
         }

@@ -542,6 +542,10 @@ int dil_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
 int dil_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
 int dil_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 int dil_memset_device(void *dst, int value, size_t bytes, void *stream);
+/* A non-blocking stream with a hardware queue of its own (an all-CU mask):
+ * callers that run batches on several streams at once (the reference's
+ * stream mode) should create them here; streams from hipStreamCreate share
+ * the runtime's few queues, where concurrent calls serialise. */
 int dil_stream_create(void **stream);
 int dil_stream_destroy(void *stream);
 int dil_stream_synchronize(void *stream);

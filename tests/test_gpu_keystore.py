@@ -126,7 +126,7 @@ def test_resident_only_context_rejects_plain_sign():
         ctx.sign([b"x"] * 4, sk)
 
 
-@pytest.mark.parametrize("mode", [2, 5])
+@pytest.mark.parametrize("mode", [2, 3, 5])
 def test_randomized_signing_matches_oracle(mode):
     """dil_sign_device_randomized with caller rhoprime: host keys and resident
     keys, both engines, equal the oracle's randomized signatures."""
