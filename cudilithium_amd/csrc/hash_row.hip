@@ -23,15 +23,6 @@ namespace dil {
 
 constexpr int ROW_ITEMS_PER_BLOCK = 16;
 
-// 4 bytes little-endian at p + off, zero past len (any alignment)
-__device__ __forceinline__ uint32_t load_half_bytes(const uint8_t *p, uint32_t off, uint32_t len) {
-    if (off + 4 <= len && ((reinterpret_cast<uintptr_t>(p) + off) & 3) == 0)
-        return *reinterpret_cast<const uint32_t *>(p + off);
-    uint32_t v = 0;
-    for (uint32_t b = 0; b < 4 && off + b < len; ++b) v |= (uint32_t)p[off + b] << (8 * b);
-    return v;
-}
-
 // a[x] holds half r.h of output word x + 5 y: lanes with word < nwords
 // store their half to dst (32-bit stores; dst 4-byte aligned)
 __device__ __forceinline__ void row_store_words(const uint32_t (&a)[5], const RowLane &r, uint32_t *dst,

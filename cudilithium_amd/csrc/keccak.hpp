@@ -328,6 +328,53 @@ __device__ __forceinline__ void shake_absorb_pair(uint32_t (&s)[25], Get get, ui
     if (h == 1) s[RW - 1] ^= 0x80000000u;
 }
 
+// 4 bytes little-endian at p + off, zero past len (any alignment)
+__device__ __forceinline__ uint32_t load_half_bytes(const uint8_t *p, uint32_t off, uint32_t len) {
+    if (off + 4 <= len && ((reinterpret_cast<uintptr_t>(p) + off) & 3) == 0)
+        return *reinterpret_cast<const uint32_t *>(p + off);
+    uint32_t v = 0;
+    for (uint32_t b = 0; b < 4 && off + b < len; ++b) v |= (uint32_t)p[off + b] << (8 * b);
+    return v;
+}
+
+// SHAKE absorb of a byte stream of run-time length `len` in the pair layout,
+// plus the first squeeze permutation: get_half(w) returns this lane's half of
+// input word w (bytes past the end zero; only called for w < ceil(len / 8)).
+// Afterwards s[w] holds half h of output word w of the first block.  Both
+// lanes of the pair run the same trip count (same item).
+template <int RATE, class GetHalf>
+__device__ __forceinline__ void shake_pair(uint32_t (&s)[25], uint32_t h, uint32_t len, GetHalf get_half) {
+    constexpr uint32_t RW = RATE / 8;
+#pragma unroll
+    for (int i = 0; i < 25; ++i) s[i] = 0u;
+    const uint32_t nwords = (len + 7) / 8, nfull = len / RATE;
+    auto fetch = [&](uint32_t blk, uint32_t i) -> uint32_t {
+        const uint32_t w = blk * RW + i;
+        return w < nwords ? get_half(w) : 0u;
+    };
+    uint32_t cur[RW];
+#pragma unroll
+    for (uint32_t i = 0; i < RW; ++i) cur[i] = fetch(0, i);
+#pragma unroll 1
+    for (uint32_t blk = 0; blk < nfull; ++blk) {
+#pragma unroll
+        for (uint32_t i = 0; i < RW; ++i) s[i] ^= cur[i];
+#pragma unroll
+        for (uint32_t i = 0; i < RW; ++i) cur[i] = fetch(blk + 1, i);  // overlaps the permutation
+        keccak_f1600_pair(s, h);
+    }
+    // last (partial) block: domain byte 0x1F at byte rem, final bit 0x80 at byte RATE - 1
+    const uint32_t rem = len - nfull * RATE;
+#pragma unroll
+    for (uint32_t i = 0; i < RW; ++i) {
+        uint32_t v = cur[i];
+        if (i == rem / 8 && h == (rem % 8) / 4) v ^= 0x1Fu << (8 * (rem % 4));
+        if (i == RW - 1 && h == 1) v ^= 0x80000000u;
+        s[i] ^= v;
+    }
+    keccak_f1600_pair(s, h);
+}
+
 // full 64-bit word i of the state (both lanes of the pair get it)
 __device__ __forceinline__ uint64_t pair_word(const uint32_t (&s)[25], int i, uint32_t h) {
     const uint32_t own = s[i], other = pair_other(own);
