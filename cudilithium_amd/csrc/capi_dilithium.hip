@@ -494,6 +494,7 @@ int dil_crypto_sign_verify(int mode, int *ret, const uint8_t *sig, size_t sig_pi
         VerifyArgs a{d_pool + VL::SIG, pitch, d_pool + VL::PK, pitch,
                      MsgRef{d_pool + VL::M, nullptr, pitch, nullptr, (uint32_t)mlen},
                      d_pool, pitch, d_ret, pitch / 4, batch};
+        a.fork_mu = verify_fork_mu(batch);
         if ((e = launch_verify(mode, a, st))) return e;
         // api.cu:513
         return copy2d(ret, sizeof(int), d_ret, pitch, sizeof(int), batch, hipMemcpyDeviceToHost, st);
@@ -586,6 +587,7 @@ int dil_verify_device_ex(int mode, int32_t *d_ret, const uint8_t *d_sig, size_t 
     if (work_pitch < dil_verify_work_row_bytes(mode) || (work_pitch & 255)) return DIL_ENOMEM;
     VerifyArgs a{d_sig, sig_pitch, d_pk, pk_pitch, MsgRef{d_msg, d_moff, 0, d_mlen, 0}, d_work, work_pitch, d_ret, 1, n};
     a.row_mu = (flags & DIL_HASH_LONG_MESSAGES) ? 1 : 0;
+    a.fork_mu = verify_fork_mu(n);
     return launch_verify(mode, a, S(stream));
 }
 
@@ -1088,6 +1090,7 @@ int dil_verify_device_resident_ex(const dil_keystore *k, const uint32_t *d_key_i
     VerifyArgs a{d_sig, sig_pitch, k->rows, k->pitch, MsgRef{d_msg, d_moff, 0, d_mlen, 0}, d_work, work_pitch,
                  d_ret, 1, n, d_key_idx, k->rows, k->pitch, (uint32_t)k->capacity};
     a.row_mu = (flags & DIL_HASH_LONG_MESSAGES) ? 1 : 0;
+    a.fork_mu = verify_fork_mu(n);
     return launch_verify(mode, a, S(stream));
 }
 

@@ -57,7 +57,22 @@ struct VerifyArgs {
     // Long messages: tr and mu are computed before vf_prep with one sponge
     // per 16-lane row (vf_mu_row_kernel) and vf_prep skips them.
     int row_mu;
+    // Otherwise, fork_mu: tr and mu are computed with two lanes per item on
+    // an auxiliary stream beside vf_prep / vf_main (only vf_final needs mu);
+    // 0 keeps them in vf_prep on the caller's stream (no auxiliary streams).
+    int fork_mu;
 };
+// fork_mu of the verification entry points for a batch of n items: on for
+// card-filling batches (65,536 Dilithium5 items 4.85 -> 4.78 ms, 10,000
+// Dilithium2 items 1.33 -> 1.26 ms per call); off below 8,192 items, where
+// the extra streams cost concurrent callers more than the overlap gives
+// (serving saturation -30 %, ten 1,000-item callers -10 %).  The entry point
+// that allocates nothing keeps 0: it creates no auxiliary streams.
+#ifndef DIL_VERIFY_FORK_MU
+#define DIL_VERIFY_FORK_MU 1
+#endif
+constexpr size_t VERIFY_FORK_MIN_ITEMS = 8192;
+inline int verify_fork_mu(size_t n) { return DIL_VERIFY_FORK_MU && n >= VERIFY_FORK_MIN_ITEMS ? 1 : 0; }
 
 // Host-side per-item state of the per-stage scheduler, laid out as the
 // reference's sign_lut_element (api.cuh:31-34).

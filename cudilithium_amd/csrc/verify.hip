@@ -311,6 +311,41 @@ bool verify_items_supported(int mode, int g) {
     }
 }
 
+// tr = SHAKE256(pk)[0:32] (or the resident key's) and mu = SHAKE256(tr || m)
+// into the work row, two lanes per item (keccak_f1600_pair), 128 items per
+// workgroup: the same bytes vf_prep writes when it hashes itself
+template <class P>
+__global__ __launch_bounds__(256) void vf_mu_pair_kernel(VerifyArgs a) {
+    using Z = Sizes<P>;
+    using VL = VerifyLayout<P>;
+    __builtin_amdgcn_s_setprio(3);
+    const size_t i = (size_t)blockIdx.x * 128 + (threadIdx.x >> 1);
+    const uint32_t h = threadIdx.x & 1;
+    if (i >= a.n) return;  // both lanes of a pair
+    if (a.key_idx && a.key_idx[i] >= a.nkeys) return;  // vf_prep rejects the item
+    const uint8_t *key = a.key_idx ? a.keys + a.key_idx[i] * a.key_pitch : nullptr;
+    uint32_t s[25], tr[4];
+    if (key) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) tr[w] = load_half_bytes(key + KeyLayout<P>::TR, 8 * w + 4 * h, SEEDBYTES);
+    } else {
+        const uint8_t *pk = a.pk + i * a.pk_pitch;
+        shake_pair<SHAKE256_RATE>(s, h, (uint32_t)Z::PK_BYTES,
+                                  [&](uint32_t w) { return load_half_bytes(pk, 8 * w + 4 * h, Z::PK_BYTES); });
+#pragma unroll
+        for (int w = 0; w < 4; ++w) tr[w] = s[w];
+    }
+    const uint8_t *m = a.m.ptr(i);
+    const uint32_t mlen = a.m.len(i);
+    shake_pair<SHAKE256_RATE>(s, h, SEEDBYTES + mlen, [&](uint32_t w) -> uint32_t {
+        if (w < 4) return w == 0 ? tr[0] : (w == 1 ? tr[1] : (w == 2 ? tr[2] : tr[3]));
+        return load_half_bytes(m, 8 * (w - 4) + 4 * h, mlen);
+    });
+    uint32_t *hash = reinterpret_cast<uint32_t *>(a.work + i * a.work_pitch + VL::HASH);
+#pragma unroll
+    for (int w = 0; w < 8; ++w) hash[2 * w + h] = s[w];
+}
+
 template <class P>
 static int verify_impl(const VerifyArgs &a, hipStream_t st, hipEvent_t *ev) {
     if (a.n == 0) return 0;
@@ -318,11 +353,28 @@ static int verify_impl(const VerifyArgs &a, hipStream_t st, hipEvent_t *ev) {
     int e;
     if (ev && (e = (int)hipEventRecord(ev[0], st))) return e;
     if (a.row_mu && (e = launch_verify_mu_row(P::MODE, a, st))) return e;
-    hipLaunchKernelGGL(vf_prep_kernel<P>, dim3(b1), dim3(256), 0, st, a);
+    // fork: mu on an auxiliary stream beside vf_prep and vf_main, joined
+    // before vf_final (event fork / join; a graph capture records branches)
+    AuxStreams *aux = nullptr;
+    std::unique_lock<std::mutex> guard;
+    VerifyArgs b = a;
+    if (!a.row_mu && a.fork_mu) {
+        aux = aux_streams(st);
+        if (!aux) return (int)hipErrorOutOfMemory;
+        guard = std::unique_lock<std::mutex>(aux->lock);
+        if ((e = (int)hipEventRecord(aux->fork, st))) return e;
+        if ((e = (int)hipStreamWaitEvent(aux->s[0], aux->fork, 0))) return e;
+        hipLaunchKernelGGL(vf_mu_pair_kernel<P>, dim3((unsigned)((a.n + 127) / 128)), dim3(256), 0, aux->s[0], a);
+        if ((e = (int)hipGetLastError())) return e;
+        if ((e = (int)hipEventRecord(aux->join[0], aux->s[0]))) return e;
+        b.row_mu = 1;  // vf_prep: mu is computed elsewhere
+    }
+    hipLaunchKernelGGL(vf_prep_kernel<P>, dim3(b1), dim3(256), 0, st, b);
     if (ev && (e = (int)hipEventRecord(ev[1], st))) return e;
     e = a.key_idx ? launch_vf_main<P, true>(a, st) : launch_vf_main<P, false>(a, st);
     if (e) return e;
     if (ev && (e = (int)hipEventRecord(ev[2], st))) return e;
+    if (aux && (e = (int)hipStreamWaitEvent(st, aux->join[0], 0))) return e;
     hipLaunchKernelGGL(vf_final_kernel<P>, dim3(b1), dim3(256), 0, st, a);
     if (ev && (e = (int)hipEventRecord(ev[3], st))) return e;
     return (int)hipGetLastError();
