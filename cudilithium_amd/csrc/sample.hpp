@@ -120,6 +120,26 @@ __device__ __forceinline__ void mask_raw(const uint64_t (&rp)[8], uint32_t nonce
     }
 }
 
+// mask_raw with two lanes per sponge (keccak_f1600_pair): lane h absorbs
+// the halves h of rhoprime's words (4-byte aligned) and writes the halves h
+// of the squeezed words; the pair writes the same 680 bytes.
+__device__ __forceinline__ void mask_raw_pair(const uint32_t *rhoprime, uint32_t nonce, uint32_t *out, uint32_t h) {
+    uint32_t s[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) s[i] = 0u;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) s[w] = rhoprime[2 * w + h];
+    // 66 input bytes: le16(nonce) and the domain byte 0x1F in the low half of
+    // word 8, the final bit in the high half of word 16
+    if (h == 0) s[8] = (nonce & 0xFFFFu) | (0x1Fu << 16);
+    if (h == 1) s[SHAKE256_RATE / 8 - 1] = 0x80000000u;
+    for (int b = 0; b < 5; ++b) {
+        keccak_f1600_pair(s, h);
+#pragma unroll
+        for (int i = 0; i < SHAKE256_RATE / 8; ++i) out[2 * (17 * b + i) + h] = s[i];
+    }
+}
+
 // SampleInBall with the polynomial held in registers as two 256-bit masks:
 // bit j of nz is set when c[j] != 0 and bit j of neg when c[j] = -1.  Each
 // step (c[i] = c[b]; c[b] = +-1) becomes mask-bit moves selected without

@@ -224,6 +224,16 @@ __device__ __forceinline__ void stage_y(const SignArgs &a, size_t item, uint32_t
              reinterpret_cast<uint64_t *>(slot + SlotLayout<P>::YRAW + l * Z::YRAW_BYTES));
 }
 
+// stage_y with two lanes per sponge (both lanes of the pair call it)
+template <class P>
+__device__ __forceinline__ void stage_y_pair(const SignArgs &a, size_t item, uint32_t nonce, int l, uint8_t *slot,
+                                             uint32_t h) {
+    using Z = Sizes<P>;
+    mask_raw_pair(reinterpret_cast<const uint32_t *>(a.rows + item * a.row_pitch + SignLayout<P>::RHOPRIME),
+                  (uint32_t)(P::L * nonce + l) & 0xFFFF,
+                  reinterpret_cast<uint32_t *>(slot + SlotLayout<P>::YRAW + l * Z::YRAW_BYTES), h);
+}
+
 template <class P>
 __device__ __forceinline__ void load_y_A(int32_t (&v)[4], const uint8_t *slot, int l) {
     using Z = Sizes<P>;
