@@ -78,6 +78,8 @@ def bench_sign(args, mode: int):
         ctx.set_tiling(args.spb, args.occupancy)
     if args.claim:
         ctx.set_claim(args.claim)
+    if args.claim_guided:
+        ctx.set_claim(ctx.claim | api.CLAIM_GUIDED)
 
     for _ in range(args.warmup):
         ctx.sign_device(d_sig, d_sk, d_msgs, api.ENGINE_PERSISTENT)
@@ -171,7 +173,7 @@ def bench_sign(args, mode: int):
     cfg = {"workload": f"dilithium{mode}_sign_{n}_msg_loguniform_32B-4KiB",
            "batch": n, "mode": mode, "engine": "persistent+graph", "slots": nslots,
            "slots_per_block": ctx.slots_per_block or "auto", "occupancy": ctx.occupancy or "auto",
-           "claim": ctx.claim or "slots",
+           "claim": (ctx.claim & 0xFFFF) or "slots", "guided_claims": bool(ctx.claim & api.CLAIM_GUIDED),
            "mean_attempts": float(attempts.mean()), "loop_phase_shares": phase_share}
     return {
         "metric": METRIC, "value": value, "unit": "signs/s", "ms_per_step": wall / args.steps * 1e3,
@@ -748,6 +750,7 @@ def main():
     ap.add_argument("--spb", type=int, default=0, help="slots per workgroup (0 = autotuned)")
     ap.add_argument("--occupancy", type=int, default=0, help="waves per SIMD of the loop kernel (0 = autotuned)")
     ap.add_argument("--claim", type=int, default=0, help="items per workgroup of the loop kernel (0 = its slots)")
+    ap.add_argument("--claim-guided", action="store_true", help="guided claims near the end of the queue")
     ap.add_argument("--check", type=int, default=128, help="items checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")

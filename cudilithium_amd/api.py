@@ -24,6 +24,8 @@ ENGINE_PERSISTENT, ENGINE_PERSTAGE = 0, 1
 STATUS_ROUND_CAP, STATUS_NO_SECRET = 1, 2
 # message hashing flag of dil_sign_ctx_set_message_hashing / dil_verify_device*_ex
 HASH_LONG_MESSAGES = 1
+# flag of dil_sign_ctx_set_claim: guided claims near the end of the queue
+CLAIM_GUIDED = 0x10000
 
 
 class SignLutElement(ctypes.Structure):  # api.cuh:31-34
@@ -347,8 +349,9 @@ class SignContext:
         self.handle = p.value
         if tuned and nslots // tuned[1] >= 1:
             self.set_tiling(tuned[1], tuned[2])
-            if tuned[3]:
-                self.set_claim(tuned[3])
+            claim = tuned[3] | (CLAIM_GUIDED if tuned[4] else 0)
+            if claim:
+                self.set_claim(claim)
 
     slots_per_block = 0
     occupancy = 0
@@ -363,6 +366,7 @@ class SignContext:
         _lib.check(L().dil_sign_ctx_set_message_hashing(self.handle, int(flags)), "set_message_hashing")
 
     def set_claim(self, items_per_block: int) -> None:
+        """items per workgroup (0: its slots), optionally | CLAIM_GUIDED."""
         _lib.check(L().dil_sign_ctx_set_claim(self.handle, int(items_per_block)), "set_claim")
         self.claim = int(items_per_block)
 

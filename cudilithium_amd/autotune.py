@@ -6,8 +6,9 @@ in cudilithium_amd/tuning.json:
 
   sign{m}:{b}     persistent signing engine: attempt slots (a multiple of the
                   batch), slots per workgroup, register budget (waves per
-                  SIMD) and items claimed per workgroup (all slots, or fewer so
-                  that each item gets several nonces per round)
+                  SIMD), items claimed per workgroup (all slots, or fewer so
+                  that each item gets several nonces per round) and guided
+                  claims near the end of the queue (on / off)
   keygen{m}:{b}   keys per workgroup of keygen's main kernel
   verify{m}:{b}   items per workgroup of verify's main kernel
   ntt:{b}         wavefronts per workgroup of the standalone NTT
@@ -69,10 +70,11 @@ def write_entry(key: str, value: dict, path: str | None = None) -> None:
 
 
 def lookup(mode: int, batch: int, path: str | None = None):
-    """(slot_factor, slots_per_block, occupancy, claim) recorded for signing
-    in this mode and batch bucket, or None."""
+    """(slot_factor, slots_per_block, occupancy, claim, guided) recorded for
+    signing in this mode and batch bucket, or None."""
     e = load_cache(path).get(f"sign{mode}:{bucket(batch)}")
-    return (e["slot_factor"], e["slots_per_block"], e.get("occupancy", 0), e.get("claim", 0)) if e else None
+    return (e["slot_factor"], e["slots_per_block"], e.get("occupancy", 0), e.get("claim", 0),
+            e.get("guided", 0)) if e else None
 
 
 def apply_cache(L, path: str | None = None) -> int:
@@ -108,7 +110,7 @@ def _time(fn, reps: int, min_seconds: float = 0.05) -> float:
 
 
 def tune_sign(mode: int, batch: int, reps: int = 3, factors=SLOT_FACTORS, spbs=SLOTS_PER_BLOCK, occs=OCCUPANCY,
-              claims=CLAIM_DIVISORS, log=print):
+              claims=CLAIM_DIVISORS, guided=(0, 1), log=print):
     from . import api, workloads
     from .device import DeviceBuffer, synchronize
 
@@ -124,10 +126,10 @@ def tune_sign(mode: int, batch: int, reps: int = 3, factors=SLOT_FACTORS, spbs=S
         for spb in spbs:
             if nslots // spb < 1:
                 continue
-            for occ, div in ((o, d) for o in occs for d in claims):
+            for occ, div, g in ((o, d, g) for o in occs for d in claims for g in guided):
                 claim = max(1, spb // div) if div > 1 else 0
                 ctx.set_tiling(spb, occ)
-                ctx.set_claim(claim)
+                ctx.set_claim(claim | (api.CLAIM_GUIDED if g else 0))
                 ctx.sign_device(d_sig, d_sk, dm)
                 synchronize()
                 ctx.enable_timing(reps)
@@ -141,6 +143,7 @@ def tune_sign(mode: int, batch: int, reps: int = 3, factors=SLOT_FACTORS, spbs=S
                 executed = ctx.executed_attempts()
                 needed = int(ctx.attempt_counts(batch).sum())
                 r = {"slot_factor": f, "nslots": nslots, "slots_per_block": spb, "occupancy": occ, "claim": claim,
+                     "guided": g,
                      "blocks": nslots // spb, "batch_ms": wall * 1e3, "loop_ms": loop,
                      "executed_attempts": executed, "needed_attempts": needed,
                      "speculative_overhead": executed / max(1, needed) - 1}
@@ -215,7 +218,8 @@ def main():
     if args.op == "sign":
         best, _ = tune_sign(args.mode, args.batch, args.reps, occs=tuple(args.occupancy), log=log)
         entry = {"slot_factor": best["slot_factor"], "slots_per_block": best["slots_per_block"],
-                 "occupancy": best["occupancy"], "claim": best["claim"], "batch_ms": best["batch_ms"]}
+                 "occupancy": best["occupancy"], "claim": best["claim"], "guided": best["guided"],
+                 "batch_ms": best["batch_ms"]}
         key = f"sign{args.mode}:{bucket(args.batch)}"
     else:
         best, _ = tune_kernel(args.op, args.mode, args.batch, args.reps, log=log)

@@ -783,7 +783,7 @@ int dil_sign_ctx_set_tiling(dil_sign_ctx *c, int slots_per_block, int occupancy,
 }
 
 int dil_sign_ctx_set_claim(dil_sign_ctx *c, int items_per_block) {
-    if (!c || items_per_block < 0) return DIL_EINVAL;
+    if (!c || items_per_block < 0 || (items_per_block & ~(DIL_CLAIM_GUIDED | 0xFFFF))) return DIL_EINVAL;
     c->claim = items_per_block;
     return 0;
 }

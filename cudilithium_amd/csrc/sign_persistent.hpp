@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "internal.hpp"
 #include "sign_stages.hpp"
 
@@ -47,12 +49,18 @@ constexpr int PERSIST_PHASES = 6;
 #define DIL_KECCAK_PRIO 2
 #endif
 constexpr int KECCAK_PRIO = DIL_KECCAK_PRIO;
+// claim argument of the guided kernel: items per workgroup (low 16 bits) and
+// the number of workgroups resident at once (high 16 bits); guided claims
+// are built for the 2-waves-per-SIMD register budget only
+constexpr int CLAIM_GUIDED_FLAG = 0x10000;  // DIL_CLAIM_GUIDED of the header
 
 // WPE: waves per SIMD the register allocation must allow (512 / WPE VGPRs)
 // claim: at most this many items owned per workgroup (1..S); below S every
 // owned item gets at least S / claim slots (nonces) per round, which trades
 // speculative attempts for fewer rounds on small batches
-template <class P, int S, int WPE>
+// GUIDED: guided claims (a separate instantiation, so that the scalar code
+// for it does not change the register allocation of the plain kernel)
+template <class P, int S, int WPE, bool GUIDED>
 __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void sg_persistent_kernel(
     SignArgs a, SlotRef sr, uint32_t *head, int claim, unsigned long long *phase_sums) {
     __shared__ ZetaLds zt;
@@ -104,7 +112,26 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
             break;
         }
         if (tid == 0) {
-            const int cap = claim > 0 && claim < S ? claim : S;
+            const int claim_items = GUIDED ? claim & 0xFFFF : claim;
+            int cap = claim_items > 0 && claim_items < S ? claim_items : S;
+            if constexpr (GUIDED) {
+                // guided claims: once fewer than cap items per resident
+                // workgroup are left in the queue, own at most remaining /
+                // resident workgroups of them (rounded up), so that the last
+                // items spread over all workgroups (each then gets more slots
+                // per round) instead of holding a few workgroups while the
+                // others exit
+                const uint32_t resident = (uint32_t)claim >> 16;
+                if (!queue_empty) {
+                    const uint32_t h0 = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const size_t rem = (size_t)h0 < a.n ? a.n - h0 : 0;
+                    if (rem < (size_t)cap * resident) {
+                        int per = 1;
+                        while ((size_t)per * resident < rem) ++per;
+                        cap = per;
+                    }
+                }
+            }
             int u = n_own;
             if (u < cap && !queue_empty) {
                 const uint32_t start = atomicAdd(head, (uint32_t)(cap - u));
@@ -242,8 +269,17 @@ static int launch_persistent_s(const SignArgs &a, const SlotRef &sr, size_t nslo
     if (blocks > 0 && blocks < nb) nb = blocks;
     if (nb < 1) return -3;
     hipLaunchKernelGGL(zero_ctl_kernel, dim3(1), dim3(1), 0, st, d_head, phase_sums);
-    hipLaunchKernelGGL((sg_persistent_kernel<P, S, WPE>), dim3(nb), dim3(PERSIST_THREADS), 0, st, a, sr, d_head,
-                       claim, phase_sums);
+    // claim: items per workgroup (low 16 bits) | workgroups resident at once
+    // (high 16 bits: the register budget allows WPE waves per SIMD)
+    if ((claim & CLAIM_GUIDED_FLAG) && WPE == 2) {
+        const int resident = std::max(1, std::min(nb, device_cu_count() * 4 * WPE / (PERSIST_THREADS / 64)));
+        const int claim_arg = (claim & 0xFFFF) | (std::min(resident, 0xFFFF) << 16);
+        hipLaunchKernelGGL((sg_persistent_kernel<P, S, WPE, WPE == 2>), dim3(nb), dim3(PERSIST_THREADS), 0, st, a,
+                           sr, d_head, claim_arg, phase_sums);
+    } else {
+        hipLaunchKernelGGL((sg_persistent_kernel<P, S, WPE, false>), dim3(nb), dim3(PERSIST_THREADS), 0, st, a, sr,
+                           d_head, claim & 0xFFFF, phase_sums);
+    }
     return (int)hipGetLastError();
 }
 

@@ -280,7 +280,12 @@ int dil_sign_ctx_set_tiling(dil_sign_ctx *ctx, int slots_per_block, int occupanc
 /* Persistent engine: at most items_per_block items owned by a workgroup at a
  * time (0 = slots per workgroup).  Below that, each owned item gets several
  * slots (consecutive nonces) per round, so small batches finish in fewer
- * rounds at the price of speculative attempts; results are unchanged. */
+ * rounds at the price of speculative attempts; results are unchanged.
+ * OR DIL_CLAIM_GUIDED into items_per_block for guided claims: once the queue
+ * holds fewer than items_per_block items per resident workgroup, a workgroup
+ * owns at most (remaining items / resident workgroups), so the batch's last
+ * items spread over all workgroups (more speculation, shorter tail). */
+#define DIL_CLAIM_GUIDED 0x10000
 int dil_sign_ctx_set_claim(dil_sign_ctx *ctx, int items_per_block);
 /* Records HIP events on the call's stream around the fixed stage and the
  * rejection loop of the next max_calls dil_sign_device calls (0 turns timing

@@ -27,16 +27,17 @@ def cache(tmp_path, monkeypatch):
 def test_sign_tuner_choice_is_looked_up_and_exact(cache):
     from cudilithium_amd import api, autotune, workloads
     mode, n = 2, 700
-    best, results = autotune.tune_sign(mode, n, reps=1, factors=(1.0, 2.0), spbs=(8,), occs=(2,), claims=(1,),
-                                       log=lambda s: None)
-    assert len(results) == 2
+    best, results = autotune.tune_sign(mode, n, reps=1, factors=(1.0, 2.0), spbs=(8,), occs=(2,), claims=(1, 2),
+                                       guided=(0, 1), log=lambda s: None)
+    assert len(results) == 8
     autotune.write_entry(f"sign{mode}:{autotune.bucket(n)}",
-                         {"slot_factor": best["slot_factor"], "slots_per_block": 8, "occupancy": 2, "claim": 0,
-                          "batch_ms": best["batch_ms"]})
+                         {"slot_factor": best["slot_factor"], "slots_per_block": 8, "occupancy": 2, "claim": 4,
+                          "guided": 1, "batch_ms": best["batch_ms"]})
     assert json.load(open(cache))[f"sign{mode}:1024"]["slot_factor"] == best["slot_factor"]
-    assert autotune.lookup(mode, n) == (best["slot_factor"], 8, 2, 0)
+    assert autotune.lookup(mode, n) == (best["slot_factor"], 8, 2, 4, 1)
     ctx = api.SignContext(mode, n)
     assert ctx.nslots == int(n * best["slot_factor"]) and ctx.slots_per_block == 8
+    assert ctx.claim == 4 | api.CLAIM_GUIDED
     seeds = workloads.xoshiro_bytes(0x77, n * 32).reshape(n, 32)
     msgs = workloads.messages_loguniform(0x78, n, 32, 4096)
     pk, sk = oracle.keypair_batch(mode, seeds)
