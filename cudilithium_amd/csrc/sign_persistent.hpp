@@ -122,7 +122,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
                 // per round) instead of holding a few workgroups while the
                 // others exit
                 const uint32_t resident = (uint32_t)claim >> 16;
-                if (!queue_empty) {
+                if (!queue_empty && resident > 0) {
                     const uint32_t h0 = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     const size_t rem = (size_t)h0 < a.n ? a.n - h0 : 0;
                     if (rem < (size_t)cap * resident) {
