@@ -121,8 +121,8 @@ __global__ __launch_bounds__(PERSIST_THREADS) __attribute__((amdgpu_waves_per_eu
                 // items spread over all workgroups (each then gets more slots
                 // per round) instead of holding a few workgroups while the
                 // others exit
-                const uint32_t resident = (uint32_t)claim >> 16;
-                if (!queue_empty && resident > 0) {
+                const uint32_t resident = (uint32_t)claim >> 16;  // >= 1: the launch passes it
+                if (!queue_empty) {
                     const uint32_t h0 = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     const size_t rem = (size_t)h0 < a.n ? a.n - h0 : 0;
                     if (rem < (size_t)cap * resident) {
@@ -296,6 +296,9 @@ static int launch_persistent_w(const SignArgs &a, const SlotRef &sr, size_t nslo
         case 1: return launch_persistent_s<P, 1, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
         case 8: return launch_persistent_s<P, 8, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
         case 16: return launch_persistent_s<P, 16, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
+#ifdef DIL_SPB24
+        case 24: return launch_persistent_s<P, 24, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
+#endif
         case 32: return launch_persistent_s<P, 32, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
         default: return -1;
     }
