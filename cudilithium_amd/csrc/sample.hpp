@@ -216,6 +216,75 @@ __device__ __forceinline__ void challenge(const uint64_t (&ct)[4], uint32_t *mas
     o[3] = make_uint4(neg[4], neg[5], neg[6], neg[7]);
 }
 
+// challenge with the squeeze in the pair layout (keccak_f1600_pair): lane h
+// of the pair holds halves h of the block, the partner's half comes by one
+// DPP move per 4 bytes consumed; both lanes sample the same polynomial and
+// only lane h = 0 stores it.  Both lanes of every pair must be active.
+template <int TAU>
+__device__ __forceinline__ void challenge_pair(const uint64_t (&ct)[4], uint32_t *masks, uint32_t hh) {
+    static_assert(N - TAU >= 192, "challenge positions i lie in mask words 6..7");
+    uint32_t nz[8], neg[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) nz[k] = neg[k] = 0;
+    uint32_t s[25];
+#pragma unroll
+    for (int w = 0; w < 25; ++w) s[w] = 0u;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) s[w] = hh ? (uint32_t)(ct[w] >> 32) : (uint32_t)ct[w];
+    if (hh == 0) s[4] = 0x1Fu;  // 32 input bytes: domain byte at byte 32
+    if (hh == 1) s[SHAKE256_RATE / 8 - 1] = 0x80000000u;
+    keccak_f1600_pair(s, hh);
+    uint64_t signs = pair_word(s, 0, hh);
+    uint32_t i = N - TAU;
+    auto step = [&](uint32_t b) {
+        if (i < (uint32_t)N && b <= i) {
+            const uint32_t wb = b >> 5, mb = 1u << (b & 31), mi = 1u << (i & 31);
+            const bool hi7 = (i >> 5) == 7;
+            // c[i] = c[b]
+            const uint32_t nzb = (mask_word(nz, wb) & mb) ? mi : 0u;
+            const uint32_t ngb = (mask_word(neg, wb) & mb) ? mi : 0u;
+            nz[6] = hi7 ? nz[6] : ((nz[6] & ~mi) | nzb);
+            nz[7] = hi7 ? ((nz[7] & ~mi) | nzb) : nz[7];
+            neg[6] = hi7 ? neg[6] : ((neg[6] & ~mi) | ngb);
+            neg[7] = hi7 ? ((neg[7] & ~mi) | ngb) : neg[7];
+            // c[b] = (-1)^(sign bit); word wb selected by a one-hot mask
+            const uint32_t sb = (uint32_t)signs & 1u ? mb : 0u, onehot = 1u << wb;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t here = 0u - ((onehot >> k) & 1u);
+                nz[k] |= mb & here;
+                neg[k] = (neg[k] & ~(mb & here)) | (sb & here);
+            }
+            signs >>= 1;
+            ++i;
+        }
+    };
+    // bytes 8.. of the first block (the first 8 are the sign bits), then
+    // further blocks if ever needed; 4 bytes per 32-bit half-word
+    // A lane is done once i reaches N (after ~55 of the 128 bytes on
+    // average); the rest of the block is skipped as soon as every active lane
+    // of the wave is done, which saves about half of the steps.
+    auto bytes_of = [&](auto h) {
+        if (__ballot(i < (uint32_t)N) == 0ull) return;  // wave-uniform: the DPP below runs in every lane
+        constexpr int J = decltype(h)::value;
+        const uint32_t own = s[J / 2], other = pair_other(own);
+        uint32_t w = (uint32_t)(J & 1) == hh ? own : other;
+#pragma unroll 1
+        for (int b = 0; b < 4; ++b, w >>= 8) step(w & 0xFFu);
+    };
+    static_for<2, SHAKE256_RATE / 4>(bytes_of);
+    while (i < (uint32_t)N) {  // same trip count in both lanes of a pair
+        keccak_f1600_pair(s, hh);
+        static_for<0, SHAKE256_RATE / 4>(bytes_of);
+    }
+    if (hh != 0) return;
+    uint4 *o = reinterpret_cast<uint4 *>(masks);
+    o[0] = make_uint4(nz[0], nz[1], nz[2], nz[3]);
+    o[1] = make_uint4(nz[4], nz[5], nz[6], nz[7]);
+    o[2] = make_uint4(neg[0], neg[1], neg[2], neg[3]);
+    o[3] = make_uint4(neg[4], neg[5], neg[6], neg[7]);
+}
+
 // Squeezes the first output block after absorption and returns words [0, NW).
 template <int NW>
 __device__ __forceinline__ void squeeze_words(Lane64 (&s)[25], uint64_t (&out)[NW]) {

@@ -296,9 +296,6 @@ static int launch_persistent_w(const SignArgs &a, const SlotRef &sr, size_t nslo
         case 1: return launch_persistent_s<P, 1, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
         case 8: return launch_persistent_s<P, 8, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
         case 16: return launch_persistent_s<P, 16, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
-#ifdef DIL_SPB24
-        case 24: return launch_persistent_s<P, 24, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
-#endif
         case 32: return launch_persistent_s<P, 32, WPE>(a, sr, nslots, blocks, claim, d_head, ph, st);
         default: return -1;
     }
