@@ -208,14 +208,16 @@ def test_dropin_persistent_matches_perstage_full_rounds():
     assert np.array_equal(a, oracle.sign_batch(2, m, sk)[0])
 
 
-@pytest.mark.parametrize("claim", [1, 3, 8])
+@pytest.mark.parametrize("claim", [1, 3, 8, 8 | 0x10000, 0x10000])
 def test_persistent_engine_claim_cap(claim):
     """Fewer items per workgroup than slots (several nonces per item per
-    round) changes no signature or attempt count."""
+    round), and guided claims (0x10000, DIL_CLAIM_GUIDED: at most remaining
+    items / resident workgroups near the end of the queue), change no
+    signature or attempt count."""
     from cudilithium_amd import api
     mode, n = 3, 150
-    _, sk = oracle.keypair_counter_batch(mode, n, rand_index=4000 + claim)
-    m = [bytes([claim]) * (i % 97) for i in range(n)]
+    _, sk = oracle.keypair_counter_batch(mode, n, rand_index=4000 + (claim & 0xFF))
+    m = [bytes([claim & 0xFF]) * (i % 97) for i in range(n)]
     exp, exp_att = oracle.sign_batch(mode, m, sk)
     ctx = api.SignContext(mode, n, 256)
     ctx.set_tiling(8, 2)
@@ -250,3 +252,21 @@ def test_verify_rejects_hint_padding_pair_like_specification():
     got = d.verify(bad, msgs[cand], pk[cand])
     assert (got == -1).all()
     assert np.array_equal(got, oracle.verify_batch(mode, bad, msgs[cand], pk[cand]))
+
+
+def test_guided_claims_across_modes_and_sizes():
+    """Guided claims at several batch sizes (one workgroup up to several
+    resident generations) and both register budgets: bytes and attempt
+    counts equal the oracle's; unknown claim flags are rejected."""
+    from cudilithium_amd import _lib, api
+    for mode, n, spb, occ in ((2, 1, 1, 2), (5, 37, 8, 2), (3, 600, 16, 2), (2, 600, 8, 3)):
+        _, sk = oracle.keypair_counter_batch(mode, n, rand_index=4100 + n)
+        m = [bytes([n & 0xFF]) * (i % 61) for i in range(n)]
+        exp, exp_att = oracle.sign_batch(mode, m, sk)
+        ctx = api.SignContext(mode, n, n)
+        ctx.set_tiling(spb, occ)
+        ctx.set_claim(2 | api.CLAIM_GUIDED)
+        assert np.array_equal(ctx.sign(m, sk), exp), (mode, n)
+        assert np.array_equal(ctx.attempt_counts(n), exp_att), (mode, n)
+    with pytest.raises(_lib.DilError):
+        ctx.set_claim(0x20000)
