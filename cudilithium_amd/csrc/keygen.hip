@@ -77,16 +77,49 @@ __global__ __launch_bounds__(256) void kg_eta_kernel(KeygenArgs a) {
     uint64_t seed[8];
 #pragma unroll
     for (int w = 0; w < 8; ++w) seed[w] = *reinterpret_cast<const uint64_t *>(rp + 8 * w);
-    // nonce l for s1[l], L + k for s2[k] (keypair.cu:84-89, 202-207)
-    eta_poly<P::ETA>(seed, (uint32_t)p, [&](int ctr, int32_t v) { s8[ctr] = (int8_t)v; });
-    // pack eta - s (keypair.cu:116-130)
+    // The coefficients arrive in order: they are collected into 16-byte
+    // stores of the int8 copy (kg_main reads it) and into
+    // the packed eta - s bit stream of the secret key (keypair.cu:116-130),
+    // written a 32-bit word at a time (bytes when the row is not 4-aligned).
+    // Lanes write different rows, so one store per coefficient made every
+    // store instruction touch 64 cache lines for a single byte each.
     uint8_t *dst = a.sk + i * a.sk_pitch + Z::SK_S1 + p * Z::POLYETA_BYTES;
-    for (int g = 0; g < N / 8; ++g) {
-        uint32_t c[8];
+    const bool dst4 = (reinterpret_cast<uintptr_t>(dst) & 3) == 0;
+    const bool s8_16 = (reinterpret_cast<uintptr_t>(s8) & 15) == 0;  // else 8-aligned (work pitch % 8 == 0)
+    uint32_t cur = 0, w0 = 0, w1 = 0, w2 = 0;  // int8 bytes: the current word and the last three
+    uint64_t bits = 0;
+    int nbits = 0, nout = 0;
+    // nonce l for s1[l], L + k for s2[k] (keypair.cu:84-89, 202-207)
+    eta_poly<P::ETA>(seed, (uint32_t)p, [&](int ctr, int32_t v) {
+        cur = (cur >> 8) | ((uint32_t)(uint8_t)(int8_t)v << 24);
+        if ((ctr & 3) == 3) {
+            if ((ctr & 15) == 15) {
+                if (s8_16) {
+                    reinterpret_cast<uint4 *>(s8)[ctr >> 4] = make_uint4(w0, w1, w2, cur);
+                } else {
+                    reinterpret_cast<uint2 *>(s8)[2 * (ctr >> 4)] = make_uint2(w0, w1);
+                    reinterpret_cast<uint2 *>(s8)[2 * (ctr >> 4) + 1] = make_uint2(w2, cur);
+                }
+            }
+            w0 = w1;
+            w1 = w2;
+            w2 = cur;
+        }
+        bits |= (uint64_t)(uint32_t)(P::ETA - v) << nbits;
+        nbits += Z::ETA_BITS;
+        if (nbits >= 32) {
+            const uint32_t word = (uint32_t)bits;
+            if (dst4) {
+                reinterpret_cast<uint32_t *>(dst)[nout] = word;
+            } else {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) c[j] = (uint32_t)(P::ETA - s8[8 * g + j]);
-        pack8_store<Z::ETA_BITS>(dst + g * Z::ETA_BITS, c);
-    }
+                for (int b = 0; b < 4; ++b) dst[4 * nout + b] = (uint8_t)(word >> (8 * b));
+            }
+            ++nout;
+            bits >>= 32;
+            nbits -= 32;
+        }
+    });
 }
 
 // keys per workgroup of kg_main: the compiled default, and the values a
@@ -190,6 +223,32 @@ __global__ __launch_bounds__(256) void kg_tr_kernel(KeygenArgs a) {
     for (int w = 0; w < 4; ++w) store_le64(sk + 8 * w, tr[w]);
 }
 
+// tr = SHAKE256(pk)[0:32] with two lanes per key (keccak_f1600_pair): the
+// serial absorb of the public key (10 / 15 / 20 permutations) is 2/3 as long
+template <class P>
+__global__ __launch_bounds__(256) void kg_tr_pair_kernel(KeygenArgs a) {
+    using Z = Sizes<P>;
+    const size_t i = (size_t)blockIdx.x * 128 + (threadIdx.x >> 1);
+    const uint32_t h = threadIdx.x & 1;
+    if (i >= a.n) return;  // both lanes of a pair
+    const uint8_t *pk = a.pk + i * a.pk_pitch;
+    uint32_t s[25];
+    shake_pair<SHAKE256_RATE>(s, h, (uint32_t)Z::PK_BYTES,
+                              [&](uint32_t w) { return load_half_bytes(pk, 8 * w + 4 * h, Z::PK_BYTES); });
+    uint8_t *sk = a.sk + i * a.sk_pitch + Z::SK_TR;
+    const bool al = (reinterpret_cast<uintptr_t>(sk) & 3) == 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        uint8_t *q = sk + 8 * w + 4 * h;
+        if (al) {
+            *reinterpret_cast<uint32_t *>(q) = s[w];
+        } else {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) q[b] = (uint8_t)(s[w] >> (8 * b));
+        }
+    }
+}
+
 template <class P, int G>
 constexpr bool kg_fits() {
     return sizeof(KgShared<P, G>) <= 160 * 1024;
@@ -226,7 +285,13 @@ static int keygen_impl(const KeygenArgs &a, hipStream_t st) {
         default: e = launch_kg_main<P, KgMainCfg<P>::G>(a, st); break;
     }
     if (e) return e;
-    hipLaunchKernelGGL(kg_tr_kernel<P>, dim3(b1), dim3(256), 0, st, a);
+#ifndef DIL_KG_TR_PAIR
+#define DIL_KG_TR_PAIR 0
+#endif
+    if (DIL_KG_TR_PAIR)
+        hipLaunchKernelGGL(kg_tr_pair_kernel<P>, dim3((unsigned)((a.n + 127) / 128)), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL(kg_tr_kernel<P>, dim3(b1), dim3(256), 0, st, a);
     return (int)hipGetLastError();
 }
 
