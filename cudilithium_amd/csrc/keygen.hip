@@ -286,7 +286,7 @@ static int keygen_impl(const KeygenArgs &a, hipStream_t st) {
     }
     if (e) return e;
 #ifndef DIL_KG_TR_PAIR
-#define DIL_KG_TR_PAIR 0
+#define DIL_KG_TR_PAIR 1  // 16,384 keys: 1-1.5 % faster keygen than one lane per key
 #endif
     if (DIL_KG_TR_PAIR)
         hipLaunchKernelGGL(kg_tr_pair_kernel<P>, dim3((unsigned)((a.n + 127) / 128)), dim3(256), 0, st, a);
