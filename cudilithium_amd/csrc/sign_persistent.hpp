@@ -5,7 +5,8 @@
 // launch in which every workgroup loops until the batch is done:
 //
 //   refill   claim items from a global queue (one atomicAdd per refill) until
-//            the workgroup owns S undone items or the queue is empty
+//            the workgroup owns `claim` (at most S) undone items or the queue
+//            is empty; with guided claims fewer near the end of the queue
 //   assign   S attempt slots round-robin over the owned items: item o gets
 //            slots o, o + U, o + 2U, ... with consecutive nonces, so freed
 //            slots turn into speculative nonces of unfinished items instead of
@@ -15,6 +16,7 @@
 //   z        one wave per item: its slots in slot order = nonce order; the
 //            first that passes is the smallest successful nonce -- the
 //            sequential specification's result -- and writes the signature
+//            (z, h and c~ straight into the item's signature row)
 //
 // Workgroups never wait on each other (the queue is only read-modify-written
 // by atomicAdd), so there is no co-residency requirement and no deadlock: a
