@@ -38,7 +38,7 @@ PEAK_HBM_GBS = 8000.0
 # written = WRITE_SIZE); profiles/sign3_16384_hbm_valu_counters.md.
 TRAFFIC_BYTES = {
     # loop kernel per launch: 2 x FETCH_SIZE + WRITE_SIZE KiB (profiles/sign3_16384_hbm_valu_counters.md)
-    "dilithium3_sign_16384_msg_loguniform_32B-4KiB": (2 * 1641841 + 1546986) * 1024,
+    "dilithium3_sign_16384_msg_loguniform_32B-4KiB": (2 * 1641558 + 1548709) * 1024,
     # vf_mu_pair + vf_prep + vf_main + vf_final per call (profiles/verify5_65536_kernel_stats.md)
     "dilithium5_verify_65536_msg_1KiB_10pct_bitflips":
         (2 * (119517 + 51717 + 247489 + 46140) + (4096 + 75070 + 65462 + 256)) * 1024,
