@@ -87,6 +87,37 @@ def test_keypair_counter_and_seed_forms_agree(mode):
 
 
 @pytest.mark.parametrize("mode", [2, 3, 5])
+def test_keypair_device_unaligned_rows(mode):
+    """Keygen into rows at odd addresses and odd pitches, with work rows at an
+    8-byte (not 16-byte) pitch: the byte and 8-byte store paths of ExpandS
+    and the packers give the same keys as packed, aligned rows."""
+    from cudilithium_amd import _lib, api
+    from cudilithium_amd.device import DeviceBuffer
+    L = api.L()
+    n = 45
+    pkb, skb, _ = api.sizes(mode)
+    seeds = np.frombuffer(hashlib.shake_128(b"unaligned rows %d" % mode).digest(32 * n), np.uint8).reshape(n, 32)
+    epk, esk = oracle.keypair_batch(mode, seeds)
+    pkp, skp, sp = pkb + 5, skb + 3, 33
+    wrow = (L.dil_keypair_work_row_bytes(mode) + 7) // 8 * 8
+    wrow += 8 if wrow % 16 == 0 else 0  # 8-aligned, not 16-aligned
+    d_pk, d_sk = DeviceBuffer(n * pkp + 16), DeviceBuffer(n * skp + 16)
+    d_seeds = DeviceBuffer(n * sp + 16)
+    host_seeds = np.zeros(n * sp + 16, np.uint8)
+    for i in range(n):
+        host_seeds[1 + i * sp:1 + i * sp + 32] = seeds[i]
+    d_seeds.upload(host_seeds)
+    d_work = DeviceBuffer(n * wrow + 256)
+    _lib.check(L.dil_keypair_device(mode, d_pk.ptr + 1, pkp, d_sk.ptr + 3, skp, d_seeds.ptr + 1, sp, 0, n,
+                                    d_work.ptr + 8, wrow, None), "keypair_device")
+    pk_all = d_pk.download((n * pkp + 16,), np.uint8)
+    sk_all = d_sk.download((n * skp + 16,), np.uint8)
+    for i in range(n):
+        assert np.array_equal(pk_all[1 + i * pkp:1 + i * pkp + pkb], epk[i]), (mode, i)
+        assert np.array_equal(sk_all[3 + i * skp:3 + i * skp + skb], esk[i]), (mode, i)
+
+
+@pytest.mark.parametrize("mode", [2, 3, 5])
 def test_verify_bit_flips_match_oracle(mode):
     from cudilithium_amd.api import Dilithium
     n = 256
