@@ -564,16 +564,23 @@ class KeyStore:
     def has_secret(self, idx: int) -> bool:
         return bool(L().dil_keystore_has_secret(self.handle, idx))
 
-    def verify(self, key_idx, sig: np.ndarray, msgs) -> np.ndarray:
+    def verify(self, key_idx, sig: np.ndarray, msgs, flags: int | None = None) -> np.ndarray:
+        """Verdicts under resident keys; flags (e.g. HASH_LONG_MESSAGES) go
+        to dil_verify_device_resident_ex, None to dil_verify_device_resident."""
         n = len(msgs)
         dm = DeviceMessages(msgs)
         d_idx = DeviceBuffer.from_array(np.ascontiguousarray(key_idx, dtype=np.uint32))
         d_sig = DeviceBuffer.from_array(np.ascontiguousarray(sig, dtype=np.uint8))
         wrow = L().dil_verify_work_row_bytes(self.mode)
         d_work, d_ret = DeviceBuffer(n * wrow), DeviceBuffer(4 * n)
-        _lib.check(L().dil_verify_device_resident(self.handle, d_idx.ptr, d_ret.ptr, d_sig.ptr, sig.shape[1],
-                                                  dm.data.ptr, dm.offs.ptr, dm.lens.ptr, n, d_work.ptr, wrow, None),
-                   "verify_device_resident")
+        if flags is None:
+            _lib.check(L().dil_verify_device_resident(self.handle, d_idx.ptr, d_ret.ptr, d_sig.ptr, sig.shape[1],
+                                                      dm.data.ptr, dm.offs.ptr, dm.lens.ptr, n, d_work.ptr, wrow,
+                                                      None), "verify_device_resident")
+        else:
+            _lib.check(L().dil_verify_device_resident_ex(self.handle, d_idx.ptr, d_ret.ptr, d_sig.ptr, sig.shape[1],
+                                                         dm.data.ptr, dm.offs.ptr, dm.lens.ptr, n, d_work.ptr, wrow,
+                                                         int(flags), None), "verify_device_resident_ex")
         return d_ret.download((n,), np.int32)
 
     def __del__(self):

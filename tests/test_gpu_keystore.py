@@ -144,3 +144,30 @@ def test_randomized_signing_matches_oracle(mode):
     ks.load(0, pk, sk)
     assert np.array_equal(ctx.sign_randomized(msgs, rnd, store=ks, key_idx=key_idx), exp)
     assert (oracle.verify_batch(mode, exp, msgs, pk[key_idx]) == 0).all()
+
+
+def test_resident_verify_large_batch_forked_hashing():
+    """Resident-key verification of more than 8,192 items, where tr and mu
+    are hashed with two lanes per item on an auxiliary stream beside the
+    main kernels (a partial last workgroup: 8,300 items): verdicts equal the
+    oracle's, flipped items -1, through the plain and the long-message
+    entry points."""
+    from cudilithium_amd import api
+    mode, nkeys, n = 2, 16, 8300
+    seeds = np.random.default_rng(77).integers(0, 256, (nkeys, 32), dtype=np.uint8)
+    epk, esk = oracle.keypair_batch(mode, seeds)
+    ks = api.KeyStore(mode, nkeys)
+    ks.keygen(0, seeds)
+    rng = np.random.default_rng(78)
+    key_idx = rng.integers(0, nkeys, n).astype(np.uint32)
+    msgs = _msgs(79, n, 0, 300)
+    ctx = api.SignContext(mode, n, resident_only=True)
+    sig = ctx.sign_resident(ks, key_idx, msgs, api.ENGINE_PERSISTENT)
+    flip = np.arange(5, n, 10)
+    for i in flip:
+        b = int(rng.integers(0, sig.shape[1] * 8))
+        sig[i, b // 8] ^= 1 << (b % 8)
+    oret = oracle.verify_batch(mode, sig, msgs, epk[key_idx])
+    assert (oret[flip] == -1).all() and (np.delete(oret, flip) == 0).all()
+    assert np.array_equal(ks.verify(key_idx, sig, msgs), oret)
+    assert np.array_equal(ks.verify(key_idx, sig, msgs, flags=api.HASH_LONG_MESSAGES), oret)
